@@ -1,0 +1,168 @@
+"""Benchmark: iterativeSMOS on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): N = 2^28 fp32
+values ~ Normal(0, 1) from the seeded generator, 1001 k ranks drawn uniformly
+from [1, N]; the whole vector is resident in HBM before timing starts.  One
+step = one multiselect call (ks in host memory, outputs copied back to host),
+as timed by the reference (DistributedSMOS/iterativeSMOSTimingFunctions.cu:142-151).
+
+Prints ONE JSON line.  value = elements/s of the linear-splitter (reference
+method) form; the learned-splitter form is reported beside it.
+
+    python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--log2n 28] [--nk 1001]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "ggms-distributed_amd"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--log2n", type=int, default=28)
+    ap.add_argument("--nk", type=int, default=1001)
+    ap.add_argument("--dist", default="normal")
+    ap.add_argument("--seed", type=int, default=20240609)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-check", action="store_true", help="skip the full-size exactness check")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU: the multi-GPU form (one histogram all-reduce "
+                                   "per pass) is described but not built; use --gpus 1"}))
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    import ggms
+
+    dev = torch.device("cuda:0")
+    n = 1 << args.log2n
+    x = ggms.generate("float32", args.dist, n, args.seed)
+    ks = ggms.generate_ks("uniform_random", args.nk, n, args.seed + 1)
+    torch.cuda.synchronize()
+    ctx = ggms.Context(0)
+
+    def timed(opts, steps, warmup):
+        for _ in range(warmup):
+            ctx.select(x, ks, "iterative", opts)
+        torch.cuda.synchronize()
+        times = []
+        t_all0 = time.perf_counter()
+        for _ in range(steps):
+            t0 = time.perf_counter()
+            out = ctx.select(x, ks, "iterative", opts)
+            times.append(time.perf_counter() - t0)
+        torch.cuda.synchronize()
+        t_all = time.perf_counter() - t_all0
+        return out, times, t_all
+
+    lin_opts = ggms.default_opts(splitter="linear")
+    out_lin, t_lin, tall_lin = timed(lin_opts, args.steps, args.warmup)
+    ms_per_step = 1000.0 * tall_lin / args.steps
+    value = n / (tall_lin / args.steps)
+
+    # profiled call: per-kernel durations from HIP events on the launch stream
+    ctx.select(x, ks, "iterative", ggms.default_opts(splitter="linear", profile=1))
+    st = ctx.stats()
+    bpe = 4
+    passes = []
+    for p in range(st["passes"]):
+        n_in = st["n_in"][p]
+        kept = st["n_kept"][p]
+        c_ms = st["count_ms"][p]
+        k_ms = st["compact_ms"][p]
+        passes.append({
+            "pass": p, "n_in": n_in, "n_kept": kept, "active": st["active"][p], "buckets": st["buckets"][p],
+            "count_ms": round(c_ms, 4), "compact_ms": round(k_ms, 4),
+            "count_GBs": round(n_in * bpe / (c_ms * 1e6), 1) if c_ms > 0 else None,
+            "compact_GBs": round((n_in + kept) * bpe / (k_ms * 1e6), 1) if k_ms > 0 else None,
+        })
+    # dominant kernel: first-pass count (algorithmic bytes = N * 4 per launch)
+    c0_ms = st["count_ms"][0]
+    achieved = n * bpe / (c0_ms * 1e6)
+    roofline = {"bound": "hbm", "kernel": "k_count0<float>", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "algorithmic_bytes_per_launch": n * bpe, "avg_launch_ms": round(c0_ms, 4)}
+
+    result = {
+        "metric": "elements/s + HBM GB/s, 2^28 fp32 1001 ks: learned vs linear splitters",
+        "value": round(value, 1),
+        "unit": "elements/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (Philox4x32-10 seeded generator, Box-Muller normal)",
+        "config": {"workload": f"iterativeSMOS, N=2^{args.log2n} fp32 {args.dist}(0,1), {args.nk} ks uniform "
+                               f"random in [1,N], linear splitter (1024 samples, 17 pivots, 8192 buckets)",
+                   "n": n, "nk": args.nk, "seed": args.seed, "alg": "iterative", "splitter": "linear"},
+        "roofline": roofline,
+        "passes": passes,
+        "kernel_ms": {k: round(v, 4) for k, v in st["kernel_ms"].items() if v > 0},
+        "step_ms_median": round(1000 * statistics.median(t_lin), 4),
+    }
+
+    # learned-splitter form (same engine, MLP pivots) when weights are present
+    try:
+        l_opts = ggms.default_opts(splitter="learned")
+        out_l, t_l, tall_l = timed(l_opts, args.steps, args.warmup)
+        result["learned"] = {"value": round(n / (tall_l / args.steps), 1),
+                             "ms_per_step": round(1000.0 * tall_l / args.steps, 4),
+                             "speedup_vs_linear": round(tall_lin / tall_l, 4),
+                             "identical_outputs": bool(np.array_equal(out_l.view(np.uint32),
+                                                                      out_lin.view(np.uint32)))}
+    except ggms.GGMSError as e:
+        result["learned"] = {"value": None, "note": str(e)}
+
+    if not args.no_check:
+        import oracle
+        want = oracle.sort_and_choose(x.cpu().numpy(), ks, threads=min(16, os.cpu_count() or 1))
+        result["exact_vs_sort_and_choose"] = bool(np.array_equal(want.view(np.uint32), out_lin.view(np.uint32)))
+
+    if not args.no_cpu:
+        import oracle
+        threads = min(16, os.cpu_count() or 1)
+        m = min(n, 1 << 26)
+        xs = x[:m].cpu().numpy()
+        ks_s = ggms.generate_ks("uniform_random", args.nk, m, args.seed + 1)
+        oracle.sort_and_choose(xs[: 1 << 20], ks_s[ks_s <= (1 << 20)], threads=threads)  # warm-up
+        tt = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            oracle.sort_and_choose(xs, ks_s, threads=threads)
+            tt.append(time.perf_counter() - t0)
+        t_nth = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            oracle.nth_multiselect(xs, ks_s, threads=threads)
+            t_nth.append(time.perf_counter() - t0)
+        result["cpu_baseline"] = {
+            "value": round(m / statistics.median(tt), 1), "unit": "elements/s", "cores": threads,
+            "kind": "port",
+            "sample": f"first 2^{m.bit_length() - 1} elements of the same input, {args.nk} uniform random ks; "
+                      f"order-preserving keys + __gnu_parallel::sort + pick; median of 3",
+            "nth_element_multiselect_value": round(m / statistics.median(t_nth), 1),
+            "host_cpus": os.cpu_count(),
+        }
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,15 @@
+// Library-internal limits shared by the host driver and the kernels.
+#pragma once
+#include <stdint.h>
+
+namespace ggms {
+
+// Largest bucket count of any pass (LDS histogram of 64 KB plus slack for the
+// first pass's edge and one-key ranges).
+constexpr uint32_t kMaxBuckets = 16384;
+// Largest number of pieces of the learned / sketch splitter.
+constexpr uint32_t kMaxPieces = 512;
+// Largest grid of the streaming kernels (one histogram row per workgroup).
+constexpr uint32_t kMaxWorkgroups = 4096;
+
+}  // namespace ggms

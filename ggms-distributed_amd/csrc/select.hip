@@ -1,0 +1,781 @@
+// Host driver and C ABI of the selection engine.
+//
+// The reference drives every iteration from the host with ~6 synchronous
+// copies per pass (DistributedSMOS/iterativeSMOS.cu:1348-1512).  Here the
+// whole pass (count, scan, findKBuckets, re-ranking, next splitter,
+// compaction, finishing) is decided on the device; the host reads back one
+// 128-byte control block per pass to size the next launch.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <vector>
+
+#include "ggms.h"
+#include "ggms_internal.hpp"
+#include "splitter.hpp"
+
+using namespace ggms;
+
+namespace ggms {
+
+const float* learned_weights_device(int device);  // mlp.hip
+
+#define HIP_TRY(expr)                                   \
+  do {                                                  \
+    hipError_t e_ = (expr);                             \
+    if (e_ != hipSuccess) {                             \
+      last_hip_error() = e_;                            \
+      return GGMS_EDEVICE;                              \
+    }                                                   \
+  } while (0)
+
+hipError_t& last_hip_error() {
+  static thread_local hipError_t e = hipSuccess;
+  return e;
+}
+
+static inline size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+struct Layout {
+  size_t off[32];
+  size_t total;
+};
+
+// Resolved tunables of one call.
+struct Config {
+  uint32_t B;          // requested buckets per pass
+  uint32_t B_cap;      // cap for passes >= 1
+  uint32_t B0_bound;   // upper bound of first-pass buckets
+  uint32_t R0_bound;   // upper bound of first-pass ranges
+  uint32_t S;          // sample size
+  uint32_t P;          // pivots (linear) or pieces (learned / sketch)
+  uint32_t E;          // edge buckets
+  uint32_t mode;
+  uint32_t NT;         // streaming workgroup size
+  uint32_t G_max;
+  uint32_t min_tile;
+  uint32_t max_passes;
+  uint32_t fin_cap;
+  uint64_t seed;
+  uint32_t force_finish;
+};
+
+static uint32_t pow2ceil_host(uint32_t v) {
+  uint32_t p = 1;
+  while (p < v) p <<= 1;
+  return p;
+}
+
+template <class K>
+static Config resolve_config(ggms_alg alg, uint64_t n, uint32_t nk, const ggms_opts& o, int cus) {
+  Config c{};
+  c.B = o.num_buckets ? o.num_buckets : 8192;
+  c.B = std::max<uint32_t>(16, std::min<uint32_t>(c.B, kMaxBuckets));
+  c.B_cap = kMaxBuckets;
+  c.mode = kModeLinear;
+  if (alg == GGMS_ALG_DISTRIBUTED_1GPU) c.mode = kModeStrided;
+  if (o.splitter == GGMS_SPLIT_LEARNED) c.mode = kModeLearned;
+  if (o.splitter == GGMS_SPLIT_SKETCH) c.mode = kModeSketch;
+  if (c.mode == kModeLinear || c.mode == kModeStrided) {
+    c.P = o.num_pivots ? o.num_pivots : 17;
+    c.P = std::max<uint32_t>(2, std::min<uint32_t>(c.P, 1024));
+    c.S = o.sample_size ? o.sample_size : 1024;
+  } else {
+    c.P = o.learned_pieces ? o.learned_pieces : 256;
+    c.P = std::max<uint32_t>(2, std::min<uint32_t>(c.P, kMaxPieces));
+    c.S = o.sample_size ? o.sample_size : 8192;
+    c.S = std::max<uint32_t>(c.S, 2 * kMlpWin);
+  }
+  c.S = std::max<uint32_t>(8, std::min<uint32_t>(c.S, sizeof(K) == 4 ? 16384u : 8192u));
+  uint32_t pieces = (c.mode == kModeLinear || c.mode == kModeStrided) ? std::max<uint32_t>(1, c.P - 1) : c.P;
+  c.E = std::max<uint32_t>(1, c.B / pieces / 8);
+  uint32_t Q = (c.mode == kModeLinear || c.mode == kModeStrided) ? c.P : c.P + 1;
+  c.R0_bound = 2 * Q + 6;
+  c.B0_bound = c.B + 2 * c.E + 4 + Q;
+  c.NT = o.block_threads ? o.block_threads : 1024;
+  if (c.NT % 64 || c.NT < 64 || c.NT > 1024) c.NT = 1024;
+  uint32_t bpc = o.blocks_per_cu ? o.blocks_per_cu : 2;
+  c.G_max = std::max<uint32_t>(1, std::min<uint32_t>(bpc * (uint32_t)cus, kMaxWorkgroups));
+  c.min_tile = 16384;
+  c.max_passes = o.max_passes ? std::min<uint32_t>(o.max_passes, GGMS_MAX_PASS_STATS) : 64;
+  c.fin_cap = sizeof(K) == 4 ? 16384 : 8192;
+  c.seed = o.seed;
+  c.force_finish = 0;
+  (void)n;
+  (void)nk;
+  return c;
+}
+
+struct Workspace {
+  void* base = nullptr;
+  size_t bytes = 0;
+};
+
+}  // namespace ggms
+
+struct ggms_ctx {
+  int device = 0;
+  int cus = 256;
+  size_t max_lds = 65536;
+  Workspace ws;
+  void* pinned = nullptr;  // host staging (ctl blocks, ks, outputs)
+  size_t pinned_bytes = 0;
+  ggms_stats stats{};
+  std::vector<hipEvent_t> events;  // timing events (opts.profile)
+  std::mutex mu;
+};
+
+namespace ggms {
+
+static ggms_status ensure_pinned(ggms_ctx* c, size_t bytes) {
+  if (c->pinned_bytes >= bytes) return GGMS_OK;
+  if (c->pinned) (void)hipHostFree(c->pinned);
+  c->pinned = nullptr;
+  c->pinned_bytes = 0;
+  size_t b = std::max<size_t>(bytes, 1 << 16);
+  if (hipHostMalloc(&c->pinned, b, hipHostMallocDefault) != hipSuccess) return GGMS_ENOMEM;
+  c->pinned_bytes = b;
+  return GGMS_OK;
+}
+
+static ggms_status ensure_ws(ggms_ctx* c, size_t bytes) {
+  if (c->ws.bytes >= bytes) return GGMS_OK;
+  if (c->ws.base) (void)hipFree(c->ws.base);
+  c->ws.base = nullptr;
+  c->ws.bytes = 0;
+  if (hipMalloc(&c->ws.base, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    return GGMS_ENOMEM;
+  }
+  c->ws.bytes = bytes;
+  return GGMS_OK;
+}
+
+// Optional per-kernel timing: a start/stop event pair around every launch, on
+// the launch stream, read after the call's final synchronisation.
+struct Prof {
+  ggms_ctx* c;
+  hipStream_t st;
+  bool on;
+  struct Rec {
+    int kind, pass;
+    size_t e0;
+  };
+  std::vector<Rec> recs;
+  size_t used = 0;
+  Prof(ggms_ctx* c_, hipStream_t st_, bool on_) : c(c_), st(st_), on(on_) {}
+  void begin(int kind, int pass) {
+    if (!on) return;
+    while (c->events.size() < used + 2) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) {
+        on = false;
+        return;
+      }
+      c->events.push_back(e);
+    }
+    (void)hipEventRecord(c->events[used], st);
+    recs.push_back({kind, pass, used});
+    used += 2;
+  }
+  void end() {
+    if (!on || recs.empty()) return;
+    (void)hipEventRecord(c->events[recs.back().e0 + 1], st);
+  }
+  void collect(ggms_stats& s) {
+    if (!on) return;
+    for (const Rec& r : recs) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, c->events[r.e0], c->events[r.e0 + 1]) != hipSuccess) continue;
+      s.kernel_ms[r.kind] += ms;
+      s.kernel_launches[r.kind] += 1;
+      if (r.pass >= 0 && r.pass < GGMS_MAX_PASS_STATS) {
+        if (r.kind == GGMS_KERNEL_COUNT0 || r.kind == GGMS_KERNEL_COUNTS) s.count_ms[r.pass] += ms;
+        if (r.kind == GGMS_KERNEL_COMPACT0 || r.kind == GGMS_KERNEL_COMPACTS) s.compact_ms[r.pass] += ms;
+      }
+    }
+  }
+};
+
+enum Buf {
+  bBufA, bBufB, bRows, bTotals, bRanges0, bRanges1, bCtl, bKs, bKRank, bKState, bKAux, bKList,
+  bSlotOfBucket, bSlotDst, bSlotBucket, bActBucket, bActCls, bActDst, bActSlot, bActKlo, bActKhi,
+  bSmallStart, bSmallLen, bSmallKStart, bSmallKFill, bWgFirst, bOut, bCount
+};
+
+template <class T>
+static Layout make_layout(uint64_t n, uint32_t nk, const Config& c) {
+  using K = typename KeyTraits<T>::K;
+  size_t sz[bCount];
+  size_t Bm = std::max<size_t>(c.B0_bound, c.B_cap) + 64;
+  size_t Rm = std::max<size_t>(c.R0_bound, Bm) + 64;
+  size_t nkp = std::max<uint32_t>(nk, 1);
+  size_t nbuf = n <= c.fin_cap ? n : n;
+  sz[bBufA] = nbuf * sizeof(K);
+  sz[bBufB] = n <= c.fin_cap ? 0 : nbuf * sizeof(K);
+  sz[bRows] = (size_t)c.G_max * Bm * 4;
+  sz[bTotals] = Bm * 4;
+  sz[bRanges0] = Rm * sizeof(Range<K>);
+  sz[bRanges1] = Rm * sizeof(Range<K>);
+  sz[bCtl] = (size_t)(c.max_passes + 2) * sizeof(PassCtl);
+  sz[bKs] = nkp * 4;
+  sz[bKRank] = nkp * 4;
+  sz[bKState] = nkp * 4;
+  sz[bKAux] = nkp * 4;
+  sz[bKList] = nkp * 4;
+  sz[bSlotOfBucket] = Bm * 2;
+  sz[bSlotDst] = Bm * 4;
+  sz[bSlotBucket] = Bm * 4;
+  sz[bActBucket] = Bm * 4;
+  sz[bActCls] = Bm * 4;
+  sz[bActDst] = Bm * 4;
+  sz[bActSlot] = Bm * 4;
+  sz[bActKlo] = Bm * sizeof(K);
+  sz[bActKhi] = Bm * sizeof(K);
+  sz[bSmallStart] = Bm * 4;
+  sz[bSmallLen] = Bm * 4;
+  sz[bSmallKStart] = (Bm + 1) * 4;
+  sz[bSmallKFill] = Bm * 4;
+  sz[bWgFirst] = (size_t)c.G_max * 4 * 2;  // double-buffered by pass parity
+  sz[bOut] = nkp * 8;
+  Layout L{};
+  size_t o = 0;
+  for (int i = 0; i < bCount; ++i) {
+    L.off[i] = o;
+    o += align256(sz[i]);
+  }
+  L.total = o;
+  return L;
+}
+
+template <class T>
+static size_t count_lds0(const Config& c) {
+  using K = typename KeyTraits<T>::K;
+  return ((size_t)c.B0_bound * 4 + 15) / 16 * 16 + (size_t)pow2ceil_host(c.R0_bound) * (sizeof(K) + 16);
+}
+
+static void set_lds_attr(const void* fn, size_t bytes) {
+  if (bytes > 65536) (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+template <class T>
+static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint64_t n64, const uint32_t* ks,
+                              bool ks_on_device, uint32_t nk, void* out, bool out_on_device, const ggms_opts& o) {
+  using K = typename KeyTraits<T>::K;
+  if (n64 == 0 || n64 > 0xFFFFFFFFull) return GGMS_EINVAL;
+  if (nk == 0) return GGMS_OK;
+  if (!d_vec || !ks || !out) return GGMS_EINVAL;
+  const uint32_t n = (uint32_t)n64;
+  hipStream_t st = (hipStream_t)o.stream;
+  Config c = resolve_config<K>(alg, n64, nk, o, ctx->cus);
+  if (c.mode == kModeLearned && !learned_weights_device(ctx->device)) return GGMS_ENOTSUP;
+  if (!ks_on_device)
+    for (uint32_t i = 0; i < nk; ++i)
+      if (ks[i] < 1 || ks[i] > n) return GGMS_EINVAL;
+
+  Layout L = make_layout<T>(n64, nk, c);
+  ggms_status s = ensure_ws(ctx, L.total);
+  if (s) return s;
+  const size_t ctlb = sizeof(PassCtl);
+  s = ensure_pinned(ctx, 4 * ctlb + (size_t)nk * 8 + 4096);
+  if (s) return s;
+  char* W = (char*)ctx->ws.base;
+  auto P = [&](int b) { return (void*)(W + L.off[b]); };
+  K* bufA = (K*)P(bBufA);
+  K* bufB = (K*)P(bBufB);
+  uint32_t* rows = (uint32_t*)P(bRows);
+  uint32_t* totals = (uint32_t*)P(bTotals);
+  Range<K>* rng[2] = {(Range<K>*)P(bRanges0), (Range<K>*)P(bRanges1)};
+  PassCtl* ctl = (PassCtl*)P(bCtl);
+  uint32_t* d_ks = (uint32_t*)P(bKs);
+  uint32_t* k_rank = (uint32_t*)P(bKRank);
+  uint32_t* k_state = (uint32_t*)P(bKState);
+  uint32_t* k_aux = (uint32_t*)P(bKAux);
+  uint32_t* klist = (uint32_t*)P(bKList);
+  T* d_out = out_on_device ? (T*)out : (T*)P(bOut);
+
+  char* H = (char*)ctx->pinned;
+  PassCtl* hctl = (PassCtl*)H;                 // two blocks: this pass, next pass
+  char* hdata = H + 4 * ctlb;                  // ks in, outputs out
+
+  // control block 0
+  PassCtl c0{};
+  c0.n = n;
+  uint32_t G0 = (uint32_t)std::min<uint64_t>(c.G_max, (n64 + c.min_tile - 1) / c.min_tile);
+  if (G0 < 1) G0 = 1;
+  uint32_t tile0 = (uint32_t)(((n64 + G0 - 1) / G0 + 63) & ~63ull);
+  G0 = (uint32_t)((n64 + tile0 - 1) / tile0);
+  c0.tile = tile0;
+  c0.G = G0;
+  c0.nan_lo_bucket = kNone32;
+  c0.nan_hi_bucket = kNone32;
+  memcpy(&hctl[2], &c0, ctlb);
+  {
+    uint32_t words = (uint32_t)((c.max_passes + 2) * ctlb / 4);
+    k_fill_u32<<<(words + 255) / 256, 256, 0, st>>>((uint32_t*)ctl, words, 0u);
+    HIP_TRY(hipGetLastError());
+  }
+  HIP_TRY(hipMemcpyAsync(ctl, &hctl[2], ctlb, hipMemcpyHostToDevice, st));
+  if (ks_on_device) {
+    HIP_TRY(hipMemcpyAsync(d_ks, ks, (size_t)nk * 4, hipMemcpyDeviceToDevice, st));
+  } else {
+    memcpy(hdata, ks, (size_t)nk * 4);
+    HIP_TRY(hipMemcpyAsync(d_ks, hdata, (size_t)nk * 4, hipMemcpyHostToDevice, st));
+  }
+  ggms_stats& stt = ctx->stats;
+  memset(&stt, 0, sizeof(stt));
+  Prof prof(ctx, st, o.profile != 0);
+  prof.begin(GGMS_KERNEL_OTHER, -1);
+  k_init_ks<<<(nk + 255) / 256, 256, 0, st>>>(d_ks, nk, n, k_rank, k_state, ctl);
+  prof.end();
+  HIP_TRY(hipGetLastError());
+
+  const uint32_t fin_grid = std::max<uint32_t>(1, std::min<uint32_t>(4 * (uint32_t)ctx->cus, 65535));
+  const size_t fin_lds = (size_t)c.fin_cap * sizeof(K);
+  set_lds_attr((const void*)k_finish<T>, fin_lds);
+
+  if (n <= c.fin_cap) {
+    prof.begin(GGMS_KERNEL_OTHER, -1);
+    k_tiny_setup<T><<<1, 1024, 0, st>>>(d_vec, n, nk, bufA, k_rank, k_state, (uint32_t*)P(bSmallStart),
+                                        (uint32_t*)P(bSmallLen), (uint32_t*)P(bSmallKStart), klist, ctl);
+    prof.end();
+    HIP_TRY(hipGetLastError());
+    prof.begin(GGMS_KERNEL_FINISH, 0);
+    k_finish<T><<<1, 1024, fin_lds, st>>>(bufA, ctl, (uint32_t*)P(bSmallStart), (uint32_t*)P(bSmallLen),
+                                          (uint32_t*)P(bSmallKStart), klist, k_rank, k_state, d_out,
+                                          c.fin_cap, &ctl->pad[0]);
+    prof.end();
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(&hctl[0], ctl, ctlb, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    if (hctl[0].bad_k) return GGMS_EINVAL;
+    if (hctl[0].pad[0]) return GGMS_EDEVICE;
+    stt.passes = 0;
+    stt.finished_segments = 1;
+  } else {
+    // first-pass splitter
+    Plan0Args pa{};
+    pa.mode = c.mode;
+    pa.S = c.S;
+    pa.P = c.P;
+    pa.B = c.B;
+    pa.E = c.E;
+    pa.strideS = 8192;
+    pa.seed = c.seed;
+    pa.mlp = c.mode == kModeLearned ? learned_weights_device(ctx->device) : nullptr;
+    size_t lds_p0 = (size_t)pow2ceil_host(c.S) * sizeof(K) + 2048 * sizeof(K);
+    set_lds_attr((const void*)k_plan0<T>, lds_p0);
+    prof.begin(GGMS_KERNEL_PLAN0, 0);
+    k_plan0<T><<<1, 1024, lds_p0, st>>>(d_vec, pa, rng[0], ctl);
+    prof.end();
+    HIP_TRY(hipGetLastError());
+
+    uint32_t G = G0, B_bound = c.B0_bound, R_bound = c.R0_bound;
+    uint32_t U_slot_bound = std::min<uint32_t>(nk, B_bound);
+    for (uint32_t pass = 0;; ++pass) {
+      PassCtl* cp = ctl + pass;
+      PassCtl* cn = ctl + pass + 1;
+      Range<K>* rc = rng[pass & 1];
+      Range<K>* rn = rng[(pass + 1) & 1];
+      K* src = (pass & 1) ? bufA : bufB;  // input of pass >= 1
+      K* dst = (pass & 1) ? bufB : bufA;  // output of this pass's compaction
+      // tile -> first range table: read by this pass's streaming kernels, written
+      // by this pass's plan for the next pass (so it must not alias)
+      uint32_t* wgf_cur = (uint32_t*)P(bWgFirst) + (size_t)(pass & 1) * c.G_max;
+      uint32_t* wgf_next = (uint32_t*)P(bWgFirst) + (size_t)((pass + 1) & 1) * c.G_max;
+      const uint32_t NT = c.NT;
+      // count
+      if (pass == 0) {
+        size_t lds = count_lds0<T>(c);
+        set_lds_attr((const void*)k_count0<T>, lds);
+        prof.begin(GGMS_KERNEL_COUNT0, (int)pass);
+        k_count0<T><<<G, NT, lds, st>>>(d_vec, rc, cp, rows);
+        prof.end();
+      } else {
+        size_t lds = (size_t)B_bound * 4;
+        set_lds_attr((const void*)k_countS<K>, lds);
+        prof.begin(GGMS_KERNEL_COUNTS, (int)pass);
+        k_countS<K><<<G, NT, lds, st>>>(src, rc, wgf_cur, cp, rows);
+        prof.end();
+      }
+      HIP_TRY(hipGetLastError());
+      prof.begin(GGMS_KERNEL_ROWSCAN, (int)pass);
+      k_rowscan<<<(B_bound + 63) / 64, 1024, 0, st>>>(rows, cp, totals);
+      prof.end();
+      HIP_TRY(hipGetLastError());
+      // plan
+      PlanArgs<K> pl{};
+      pl.totals = totals;
+      pl.ranges = rc;
+      pl.ctl = cp;
+      pl.ctl_next = cn;
+      pl.next_ranges = rn;
+      pl.nk = nk;
+      pl.k_rank = k_rank;
+      pl.k_state = k_state;
+      pl.k_aux = k_aux;
+      pl.slot_of_bucket = (uint16_t*)P(bSlotOfBucket);
+      pl.slot_dst = (uint32_t*)P(bSlotDst);
+      pl.slot_bucket = (uint32_t*)P(bSlotBucket);
+      pl.act_bucket = (uint32_t*)P(bActBucket);
+      pl.act_cls = (uint32_t*)P(bActCls);
+      pl.act_dst = (uint32_t*)P(bActDst);
+      pl.act_slot = (uint32_t*)P(bActSlot);
+      pl.act_klo = (K*)P(bActKlo);
+      pl.act_khi = (K*)P(bActKhi);
+      pl.small_start = (uint32_t*)P(bSmallStart);
+      pl.small_len = (uint32_t*)P(bSmallLen);
+      pl.small_kstart = (uint32_t*)P(bSmallKStart);
+      pl.small_kfill = (uint32_t*)P(bSmallKFill);
+      pl.klist = klist;
+      pl.wg_first_next = wgf_next;
+      pl.out = d_out;
+      pl.fin_cap = c.fin_cap;
+      pl.B_req = c.B;
+      pl.B_cap = c.B_cap;
+      pl.G_max = c.G_max;
+      pl.min_tile = c.min_tile;
+      pl.force_finish = (pass + 1 >= c.max_passes) ? 1u : c.force_finish;
+      size_t lds_plan = (size_t)B_bound * 8;
+      set_lds_attr((const void*)k_plan<T>, lds_plan);
+      prof.begin(GGMS_KERNEL_PLAN, (int)pass);
+      k_plan<T><<<1, 1024, lds_plan, st>>>(pl);
+      prof.end();
+      HIP_TRY(hipGetLastError());
+      // compaction
+      if (pass == 0) {
+        size_t lds = ((size_t)B_bound * 2 + 15) / 16 * 16 + (size_t)(U_slot_bound + 4) * 4 +
+                     (size_t)pow2ceil_host(R_bound) * (sizeof(K) + 16);
+        set_lds_attr((const void*)k_compact0<T>, lds);
+        prof.begin(GGMS_KERNEL_COMPACT0, (int)pass);
+        k_compact0<T><<<G, NT, lds, st>>>(d_vec, rc, cp, rows, pl.slot_of_bucket, pl.slot_dst, pl.slot_bucket, dst);
+        prof.end();
+      } else {
+        size_t lds = ((size_t)B_bound * 2 + 15) / 16 * 16 + (size_t)(U_slot_bound + 4) * 4;
+        set_lds_attr((const void*)k_compactS<K>, lds);
+        prof.begin(GGMS_KERNEL_COMPACTS, (int)pass);
+        k_compactS<K><<<G, NT, lds, st>>>(src, rc, wgf_cur, cp, rows, pl.slot_of_bucket,
+                                          pl.slot_dst, pl.slot_bucket, dst);
+        prof.end();
+      }
+      HIP_TRY(hipGetLastError());
+      prof.begin(GGMS_KERNEL_FINISH, (int)pass);
+      k_finish<T><<<fin_grid, 1024, fin_lds, st>>>(dst, cp, pl.small_start, pl.small_len, pl.small_kstart, klist,
+                                                   k_rank, k_state, d_out, c.fin_cap, &cp->pad[0]);
+      prof.end();
+      HIP_TRY(hipGetLastError());
+      HIP_TRY(hipMemcpyAsync(&hctl[0], cp, 2 * ctlb, hipMemcpyDeviceToHost, st));
+      HIP_TRY(hipStreamSynchronize(st));
+      const PassCtl& h = hctl[0];
+      const PassCtl& hn = hctl[1];
+      if (pass == 0) {
+        if (h.bad_k) return GGMS_EINVAL;
+        if (h.nan_count) return GGMS_ENAN;
+      }
+      if (pass < GGMS_MAX_PASS_STATS) {
+        stt.n_in[pass] = h.n;
+        stt.active[pass] = h.U;
+        stt.n_kept[pass] = (uint64_t)h.N_big + h.N_small;
+        stt.buckets[pass] = h.B;
+      }
+      stt.passes = pass + 1;
+      stt.finished_segments += h.U_small;
+      if (h.pad[0]) return GGMS_EDEVICE;
+      if (h.N_big == 0 || h.U_big == 0) break;
+      if (pass + 1 >= c.max_passes) return GGMS_EDEVICE;  // cannot happen: the last pass forces finishing
+      G = hn.G;
+      B_bound = std::max<uint32_t>(hn.B, 1);
+      R_bound = hn.R;
+      U_slot_bound = std::min<uint32_t>(nk, B_bound);
+    }
+  }
+  if (!out_on_device) {
+    HIP_TRY(hipMemcpyAsync(hdata, d_out, (size_t)nk * sizeof(T), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    memcpy(out, hdata, (size_t)nk * sizeof(T));
+  } else {
+    HIP_TRY(hipStreamSynchronize(st));
+  }
+  prof.collect(stt);
+  return GGMS_OK;
+}
+
+template <class T>
+static ggms_status run_first_pass(ggms_ctx* ctx, const T* d_vec, uint64_t n64, const ggms_opts& o,
+                                  ggms_range_u64* ranges_host, uint32_t* num_ranges, uint32_t* hist_host,
+                                  uint32_t* num_buckets) {
+  using K = typename KeyTraits<T>::K;
+  if (n64 == 0 || n64 > 0xFFFFFFFFull || !d_vec || !num_ranges || !num_buckets) return GGMS_EINVAL;
+  const uint32_t n = (uint32_t)n64;
+  hipStream_t st = (hipStream_t)o.stream;
+  Config c = resolve_config<K>(GGMS_ALG_ITERATIVE, n64, 1, o, ctx->cus);
+  if (c.mode == kModeLearned && !learned_weights_device(ctx->device)) return GGMS_ENOTSUP;
+  Layout L = make_layout<T>(n64, 1, c);
+  ggms_status s = ensure_ws(ctx, L.total);
+  if (s) return s;
+  const size_t ctlb = sizeof(PassCtl);
+  size_t rbytes = (size_t)c.R0_bound * sizeof(Range<K>);
+  s = ensure_pinned(ctx, 4 * ctlb + rbytes + (size_t)c.B0_bound * 4 + 4096);
+  if (s) return s;
+  char* W = (char*)ctx->ws.base;
+  auto P = [&](int b) { return (void*)(W + L.off[b]); };
+  uint32_t* rows = (uint32_t*)P(bRows);
+  uint32_t* totals = (uint32_t*)P(bTotals);
+  Range<K>* rng = (Range<K>*)P(bRanges0);
+  PassCtl* ctl = (PassCtl*)P(bCtl);
+  char* H = (char*)ctx->pinned;
+  PassCtl* hctl = (PassCtl*)H;
+  char* hdata = H + 4 * ctlb;
+  PassCtl c0{};
+  c0.n = n;
+  uint32_t G0 = (uint32_t)std::min<uint64_t>(c.G_max, (n64 + c.min_tile - 1) / c.min_tile);
+  if (G0 < 1) G0 = 1;
+  uint32_t tile0 = (uint32_t)(((n64 + G0 - 1) / G0 + 63) & ~63ull);
+  G0 = (uint32_t)((n64 + tile0 - 1) / tile0);
+  c0.tile = tile0;
+  c0.G = G0;
+  c0.nan_lo_bucket = kNone32;
+  c0.nan_hi_bucket = kNone32;
+  memcpy(&hctl[2], &c0, ctlb);
+  HIP_TRY(hipMemcpyAsync(ctl, &hctl[2], ctlb, hipMemcpyHostToDevice, st));
+  Plan0Args pa{};
+  pa.mode = c.mode;
+  pa.S = c.S;
+  pa.P = c.P;
+  pa.B = c.B;
+  pa.E = c.E;
+  pa.strideS = 8192;
+  pa.seed = c.seed;
+  pa.mlp = c.mode == kModeLearned ? learned_weights_device(ctx->device) : nullptr;
+  size_t lds_p0 = (size_t)pow2ceil_host(c.S) * sizeof(K) + 2048 * sizeof(K);
+  set_lds_attr((const void*)k_plan0<T>, lds_p0);
+  k_plan0<T><<<1, 1024, lds_p0, st>>>(d_vec, pa, rng, ctl);
+  HIP_TRY(hipGetLastError());
+  size_t lds = count_lds0<T>(c);
+  set_lds_attr((const void*)k_count0<T>, lds);
+  k_count0<T><<<G0, c.NT, lds, st>>>(d_vec, rng, ctl, rows);
+  HIP_TRY(hipGetLastError());
+  k_rowscan<<<(c.B0_bound + 63) / 64, 1024, 0, st>>>(rows, ctl, totals);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(&hctl[0], ctl, ctlb, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  const uint32_t R = hctl[0].R, B = hctl[0].B;
+  if (*num_ranges < R || *num_buckets < B) {
+    *num_ranges = R;
+    *num_buckets = B;
+    return GGMS_EINVAL;
+  }
+  Range<K>* hr = (Range<K>*)hdata;
+  uint32_t* hh = (uint32_t*)(hdata + rbytes);
+  HIP_TRY(hipMemcpyAsync(hr, rng, (size_t)R * sizeof(Range<K>), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(hh, totals, (size_t)B * 4, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  for (uint32_t i = 0; i < R; ++i) {
+    ranges_host[i].lo = hr[i].lo;
+    ranges_host[i].hi = hr[i].hi;
+    ranges_host[i].base = hr[i].base;
+    ranges_host[i].nb = hr[i].nb;
+    ranges_host[i].sh = hr[i].sh;
+    ranges_host[i].mult = hr[i].mult;
+  }
+  if (hist_host) memcpy(hist_host, hh, (size_t)B * 4);
+  *num_ranges = R;
+  *num_buckets = B;
+  return GGMS_OK;
+}
+
+// Default contexts, one per device, for the reference-shaped wrappers.
+static std::mutex g_default_mu;
+static std::vector<ggms_ctx*> g_default;
+
+static ggms_ctx* default_ctx() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+  std::lock_guard<std::mutex> lk(g_default_mu);
+  if ((int)g_default.size() <= dev) g_default.resize(dev + 1, nullptr);
+  if (!g_default[dev]) {
+    ggms_ctx* c = nullptr;
+    if (ggms_ctx_create(dev, &c) != GGMS_OK) return nullptr;
+    g_default[dev] = c;
+  }
+  return g_default[dev];
+}
+
+template <class T>
+static int wrapper(ggms_alg alg, ggms_dtype dt, T* d_vector, int length, const uint32_t* kVals_ori, int numKs,
+                   T* outputs, int blocks, int threads) {
+  (void)blocks;
+  if (length <= 0 || numKs < 0) return GGMS_EINVAL;
+  ggms_ctx* c = default_ctx();
+  if (!c) return GGMS_EDEVICE;
+  ggms_opts o;
+  ggms_opts_default(&o);
+  if (threads > 0 && threads % 64 == 0 && threads <= 1024) o.block_threads = (uint32_t)threads;
+  return ggms_select(c, alg, dt, d_vector, (uint64_t)length, kVals_ori, (uint32_t)numKs, outputs, &o);
+}
+
+}  // namespace ggms
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+void ggms_opts_default(ggms_opts* o) {
+  if (!o) return;
+  memset(o, 0, sizeof(*o));
+  o->num_buckets = 8192;
+  o->num_pivots = 17;
+  o->sample_size = 0;
+  o->block_threads = 1024;
+  o->blocks_per_cu = 2;
+  o->splitter = GGMS_SPLIT_LINEAR;
+  o->learned_pieces = 256;
+  o->max_passes = 64;
+  o->seed = 0x5EEDull;
+  o->stream = nullptr;
+}
+
+const char* ggms_status_str(ggms_status s) {
+  switch (s) {
+    case GGMS_OK: return "GGMS_OK";
+    case GGMS_EINVAL: return "GGMS_EINVAL";
+    case GGMS_ENAN: return "GGMS_ENAN";
+    case GGMS_ENOMEM: return "GGMS_ENOMEM";
+    case GGMS_EDEVICE: return "GGMS_EDEVICE";
+    case GGMS_ENOTSUP: return "GGMS_ENOTSUP";
+  }
+  return "GGMS_UNKNOWN";
+}
+
+ggms_status ggms_ctx_create(int device, ggms_ctx** out) {
+  if (!out) return GGMS_EINVAL;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return GGMS_EDEVICE;
+  ggms_ctx* c = new (std::nothrow) ggms_ctx();
+  if (!c) return GGMS_ENOMEM;
+  c->device = device;
+  int cur = 0;
+  (void)hipGetDevice(&cur);
+  if (hipSetDevice(device) != hipSuccess) {
+    delete c;
+    return GGMS_EDEVICE;
+  }
+  int cus = 0, lds = 0;
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+  (void)hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device);
+  c->cus = cus > 0 ? cus : 256;
+  c->max_lds = lds > 0 ? (size_t)lds : 65536;
+  (void)hipSetDevice(cur);
+  *out = c;
+  return GGMS_OK;
+}
+
+void ggms_ctx_destroy(ggms_ctx* c) {
+  if (!c) return;
+  int cur = 0;
+  (void)hipGetDevice(&cur);
+  (void)hipSetDevice(c->device);
+  if (c->ws.base) (void)hipFree(c->ws.base);
+  if (c->pinned) (void)hipHostFree(c->pinned);
+  for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
+  (void)hipSetDevice(cur);
+  delete c;
+}
+
+ggms_status ggms_ctx_stats(const ggms_ctx* c, ggms_stats* out) {
+  if (!c || !out) return GGMS_EINVAL;
+  *out = c->stats;
+  return GGMS_OK;
+}
+
+uint64_t ggms_workspace_bytes(ggms_dtype t, uint64_t n, uint32_t nk, const ggms_opts* opts) {
+  ggms_opts o;
+  if (opts) o = *opts; else ggms_opts_default(&o);
+  int cus = 256;
+  switch (t) {
+    case GGMS_F32: return make_layout<float>(n, nk, resolve_config<uint32_t>(GGMS_ALG_ITERATIVE, n, nk, o, cus)).total;
+    case GGMS_F64: return make_layout<double>(n, nk, resolve_config<uint64_t>(GGMS_ALG_ITERATIVE, n, nk, o, cus)).total;
+    case GGMS_U32: return make_layout<uint32_t>(n, nk, resolve_config<uint32_t>(GGMS_ALG_ITERATIVE, n, nk, o, cus)).total;
+    case GGMS_I32: return make_layout<int32_t>(n, nk, resolve_config<uint32_t>(GGMS_ALG_ITERATIVE, n, nk, o, cus)).total;
+  }
+  return 0;
+}
+
+static ggms_status select_common(ggms_ctx* ctx, ggms_alg alg, ggms_dtype t, const void* d_vec, uint64_t n,
+                                 const uint32_t* ks, bool ks_dev, uint32_t nk, void* out, bool out_dev,
+                                 const ggms_opts* opts) {
+  if (!ctx) return GGMS_EINVAL;
+  if ((unsigned)alg > 2u) return GGMS_EINVAL;
+  ggms_opts o;
+  if (opts) o = *opts; else ggms_opts_default(&o);
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  int cur = 0;
+  (void)hipGetDevice(&cur);
+  if (cur != ctx->device) (void)hipSetDevice(ctx->device);
+  ggms_status s;
+  switch (t) {
+    case GGMS_F32: s = run_select<float>(ctx, alg, (const float*)d_vec, n, ks, ks_dev, nk, out, out_dev, o); break;
+    case GGMS_F64: s = run_select<double>(ctx, alg, (const double*)d_vec, n, ks, ks_dev, nk, out, out_dev, o); break;
+    case GGMS_U32: s = run_select<uint32_t>(ctx, alg, (const uint32_t*)d_vec, n, ks, ks_dev, nk, out, out_dev, o); break;
+    case GGMS_I32: s = run_select<int32_t>(ctx, alg, (const int32_t*)d_vec, n, ks, ks_dev, nk, out, out_dev, o); break;
+    default: s = GGMS_EINVAL;
+  }
+  if (cur != ctx->device) (void)hipSetDevice(cur);
+  return s;
+}
+
+ggms_status ggms_select(ggms_ctx* ctx, ggms_alg alg, ggms_dtype t, const void* d_vec, uint64_t n,
+                        const uint32_t* ks_host, uint32_t nk, void* out_host, const ggms_opts* opts) {
+  return select_common(ctx, alg, t, d_vec, n, ks_host, false, nk, out_host, false, opts);
+}
+
+ggms_status ggms_select_device(ggms_ctx* ctx, ggms_alg alg, ggms_dtype t, const void* d_vec, uint64_t n,
+                               const uint32_t* d_ks, uint32_t nk, void* d_out, const ggms_opts* opts) {
+  return select_common(ctx, alg, t, d_vec, n, d_ks, true, nk, d_out, true, opts);
+}
+
+ggms_status ggms_debug_first_pass(ggms_ctx* ctx, ggms_dtype t, const void* d_vec, uint64_t n, const ggms_opts* opts,
+                                  ggms_range_u64* ranges_host, uint32_t* num_ranges, uint32_t* hist_host,
+                                  uint32_t* num_buckets) {
+  if (!ctx) return GGMS_EINVAL;
+  ggms_opts o;
+  if (opts) o = *opts; else ggms_opts_default(&o);
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  switch (t) {
+    case GGMS_F32: return run_first_pass<float>(ctx, (const float*)d_vec, n, o, ranges_host, num_ranges, hist_host, num_buckets);
+    case GGMS_F64: return run_first_pass<double>(ctx, (const double*)d_vec, n, o, ranges_host, num_ranges, hist_host, num_buckets);
+    case GGMS_U32: return run_first_pass<uint32_t>(ctx, (const uint32_t*)d_vec, n, o, ranges_host, num_ranges, hist_host, num_buckets);
+    case GGMS_I32: return run_first_pass<int32_t>(ctx, (const int32_t*)d_vec, n, o, ranges_host, num_ranges, hist_host, num_buckets);
+  }
+  return GGMS_EINVAL;
+}
+
+#define GGMS_WRAPPERS(SUF, TY, DT)                                                                              \
+  int ggms_iterativeSMOSWrapper_##SUF(TY* d_vector, int length, const uint32_t* kVals_ori, int numKs,         \
+                                      TY* outputs, int blocks, int threads) {                                   \
+    return wrapper<TY>(GGMS_ALG_ITERATIVE, DT, d_vector, length, kVals_ori, numKs, outputs, blocks, threads);   \
+  }                                                                                                             \
+  int ggms_bucketMultiselectWrapper_##SUF(TY* d_vector, int length, const uint32_t* kVals_ori, int numKs,     \
+                                          TY* outputs, int blocks, int threads) {                               \
+    return wrapper<TY>(GGMS_ALG_BUCKET, DT, d_vector, length, kVals_ori, numKs, outputs, blocks, threads);      \
+  }                                                                                                             \
+  int ggms_distributedSMOSWrapper_##SUF(TY* d_vector, int length, const uint32_t* kVals_ori, int numKs,       \
+                                        TY* outputs, int blocks, int threads, int rank) {                       \
+    (void)rank;                                                                                                 \
+    return wrapper<TY>(GGMS_ALG_DISTRIBUTED_1GPU, DT, d_vector, length, kVals_ori, numKs, outputs, blocks,      \
+                       threads);                                                                                \
+  }
+
+GGMS_WRAPPERS(f32, float, GGMS_F32)
+GGMS_WRAPPERS(f64, double, GGMS_F64)
+GGMS_WRAPPERS(u32, uint32_t, GGMS_U32)
+GGMS_WRAPPERS(i32, int32_t, GGMS_I32)
+
+}  // extern "C"

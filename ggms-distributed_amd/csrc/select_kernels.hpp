@@ -1,0 +1,850 @@
+// Selection kernels for gfx950 (wave64, 160 KB LDS per CU).
+//
+// One bucketing pass = count -> rowscan -> plan -> compact -> finish:
+//   k_plan0    one workgroup: sample, sort the sample in LDS, place pivots,
+//              build the first pass's range table (linear-slope splitter).
+//   k_count0 / k_countS
+//              fused "assign bucket + count": 128-bit loads, per-workgroup LDS
+//              histogram, wave-peeled atomics; writes one histogram row per
+//              workgroup.  No per-element bucket ids are stored.
+//   k_rowscan  column-wise exclusive prefix over the rows (per-workgroup write
+//              offsets) plus the column totals (the pass histogram).
+//   k_plan     one workgroup: inclusive scan of the histogram, findKBuckets for
+//              every live k, dedupe into active buckets, classify each active
+//              bucket (single key / small / big), lay out the next buffer,
+//              re-rank ks, and build the next pass's ranges.
+//   k_compact0 / k_compactS
+//              recompute each element's bucket and copy active-bucket elements
+//              to their segment, at the workgroup's precomputed offset.
+//   k_finish   one workgroup per small segment: bitonic sort in LDS and pick.
+//
+// Reference counterparts (DistributedSMOS/): assignSmartBucket_iterative
+// iterativeSMOS.cu:488-571, sumCounts :580-586, findKBuckets :207-232,
+// updatekVals_iterative :302-376, reindexCounts :593-605,
+// copyElements_iterative :611-656, updatePivots_iterative :715-758,
+// generateBucketsandSlopes_iterative :404-477, generatePivots :1011-1054.
+#pragma once
+#include "ggms_device.hpp"
+#include "philox.hpp"
+
+namespace ggms {
+
+constexpr uint32_t kNone16 = 0xFFFFu;
+constexpr uint32_t kNone32 = 0xFFFFFFFFu;
+
+enum KState : uint32_t { kLive = 0, kSmall = 1, kDone = 2, kSmallNew = 3 };
+enum ActClass : uint32_t { kResolved = 0, kClsSmall = 1, kClsBig = 2 };
+enum SplitMode : uint32_t { kModeLinear = 0, kModeLearned = 1, kModeSketch = 2, kModeStrided = 3 };
+
+// Device-side control block of one pass.  The plan kernel of pass p fills the
+// block of pass p+1; kernels read their own block, so the host only needs
+// upper bounds for launch shapes.
+struct PassCtl {
+  uint32_t n;        // elements entering the pass (pass >= 1: big-segment elements)
+  uint32_t tile;     // elements per workgroup
+  uint32_t G;        // workgroups that own a tile
+  uint32_t R;        // ranges of the pass
+  uint32_t Rp;       // R rounded up to a power of two (pass-0 search)
+  uint32_t B;        // buckets of the pass
+  uint32_t U;        // active buckets found by the plan of this pass
+  uint32_t U_big;    // big active buckets (ranges of the next pass)
+  uint32_t U_small;  // small active buckets (finished by LDS sort)
+  uint32_t U_slots;  // U_big + U_small
+  uint32_t N_big;    // elements of big segments (next pass input)
+  uint32_t N_small;  // elements of small segments
+  uint32_t nan_count;
+  uint32_t live_ks;
+  uint32_t nan_lo_bucket;  // bucket holding keys below key(-inf), or kNone32
+  uint32_t nan_hi_bucket;  // bucket holding keys above key(+inf), or kNone32
+  uint32_t bad_k;          // number of ks outside [1, n]
+  uint32_t pad[15];
+};
+
+struct Plan0Args {
+  uint32_t mode;        // SplitMode
+  uint32_t S;           // sample size (<= 8192)
+  uint32_t P;           // linear: number of pivots; learned/sketch: pieces
+  uint32_t B;           // bucket budget
+  uint32_t E;           // buckets of each edge range (below sample min / above sample max)
+  uint32_t strideS;     // strided mode: first-level strided sample size (8192)
+  uint64_t seed;
+  const float* mlp;     // learned mode: MLP parameters (see mlp layout in select.hip)
+};
+
+// ---------------------------------------------------------------------------
+// workgroup utilities
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t wave_incl_scan_u32(uint32_t v) {
+  const uint32_t lane = lane_id();
+#pragma unroll
+  for (uint32_t d = 1; d < 64; d <<= 1) {
+    uint32_t t = __shfl_up(v, d, 64);
+    if (lane >= d) v += t;
+  }
+  return v;
+}
+
+// Exclusive scan of one value per thread over the whole workgroup.
+// scratch: >= 16 u32 of LDS.  Returns exclusive prefix, total in *total.
+__device__ __forceinline__ uint32_t block_excl_scan_u32(uint32_t v, uint32_t* scratch, uint32_t* total) {
+  const uint32_t lane = lane_id(), wv = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+  uint32_t inc = wave_incl_scan_u32(v);
+  __syncthreads();
+  if (lane == 63) scratch[wv] = inc;
+  __syncthreads();
+  uint32_t off = 0, tot = 0;
+  for (uint32_t w = 0; w < nw; ++w) {
+    uint32_t s = scratch[w];
+    if (w < wv) off += s;
+    tot += s;
+  }
+  *total = tot;
+  return off + inc - v;
+}
+
+// In-place inclusive scan of an LDS array of length L.  scratch >= 16 u32.
+__device__ void block_incl_scan_lds(uint32_t* a, uint32_t L, uint32_t* scratch) {
+  const uint32_t nt = blockDim.x, t = threadIdx.x;
+  uint32_t per = (L + nt - 1) / nt;
+  uint32_t b0 = t * per, b1 = b0 + per < L ? b0 + per : L;
+  uint32_t s = 0;
+  for (uint32_t i = b0; i < b1; ++i) s += a[i];
+  uint32_t tot;
+  uint32_t off = block_excl_scan_u32(s, scratch, &tot);
+  for (uint32_t i = b0; i < b1; ++i) {
+    off += a[i];
+    a[i] = off;
+  }
+  __syncthreads();
+}
+
+template <class K>
+__device__ void bitonic_sort_lds(K* s, uint32_t M) {
+  for (uint32_t size = 2; size <= M; size <<= 1) {
+    for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+      for (uint32_t i = threadIdx.x; i < (M >> 1); i += blockDim.x) {
+        uint32_t lo = 2 * i - (i & (stride - 1));
+        uint32_t hi = lo + stride;
+        bool asc = (lo & size) == 0;
+        K a = s[lo], b = s[hi];
+        if ((a > b) == asc) {
+          s[lo] = b;
+          s[hi] = a;
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+__device__ __forceinline__ uint32_t pow2ceil_u32(uint32_t v) {
+  if (v <= 1) return 1;
+  return 1u << (32 - __builtin_clz(v - 1));
+}
+
+// ---------------------------------------------------------------------------
+// vector loads: 16 bytes per lane
+// ---------------------------------------------------------------------------
+template <class V> struct Vec16;
+template <> struct Vec16<float> {
+  static constexpr int N = 4;
+  __device__ static void unpack(const uint4& q, float* o) {
+    o[0] = __builtin_bit_cast(float, q.x); o[1] = __builtin_bit_cast(float, q.y);
+    o[2] = __builtin_bit_cast(float, q.z); o[3] = __builtin_bit_cast(float, q.w);
+  }
+};
+template <> struct Vec16<uint32_t> {
+  static constexpr int N = 4;
+  __device__ static void unpack(const uint4& q, uint32_t* o) { o[0] = q.x; o[1] = q.y; o[2] = q.z; o[3] = q.w; }
+};
+template <> struct Vec16<int32_t> {
+  static constexpr int N = 4;
+  __device__ static void unpack(const uint4& q, int32_t* o) {
+    o[0] = (int32_t)q.x; o[1] = (int32_t)q.y; o[2] = (int32_t)q.z; o[3] = (int32_t)q.w;
+  }
+};
+template <> struct Vec16<double> {
+  static constexpr int N = 2;
+  __device__ static void unpack(const uint4& q, double* o) {
+    o[0] = __builtin_bit_cast(double, ((uint64_t)q.y << 32) | q.x);
+    o[1] = __builtin_bit_cast(double, ((uint64_t)q.w << 32) | q.z);
+  }
+};
+template <> struct Vec16<uint64_t> {
+  static constexpr int N = 2;
+  __device__ static void unpack(const uint4& q, uint64_t* o) {
+    o[0] = ((uint64_t)q.y << 32) | q.x;
+    o[1] = ((uint64_t)q.w << 32) | q.z;
+  }
+};
+
+// Streams elements [t0, t1) of x (16-byte aligned base, t0 a multiple of the
+// vector width) through f(value, position).  Four 16-byte loads in flight per
+// lane.
+template <class V, class F>
+__device__ __forceinline__ void stream_tile(const V* __restrict__ x, uint32_t t0, uint32_t t1, F&& f) {
+  constexpr int N = Vec16<V>::N;
+  const uint32_t nt = blockDim.x;
+  const uint4* xv = reinterpret_cast<const uint4*>(x);
+  uint32_t v1 = t1 / N;
+  uint32_t v = t0 / N + threadIdx.x;
+  for (; v + 3 * nt < v1; v += 4 * nt) {
+    uint4 q0 = xv[v];
+    uint4 q1 = xv[v + nt];
+    uint4 q2 = xv[v + 2 * nt];
+    uint4 q3 = xv[v + 3 * nt];
+    V e[4][N];
+    Vec16<V>::unpack(q0, e[0]);
+    Vec16<V>::unpack(q1, e[1]);
+    Vec16<V>::unpack(q2, e[2]);
+    Vec16<V>::unpack(q3, e[3]);
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int j = 0; j < N; ++j) f(e[u][j], (v + u * nt) * N + j);
+  }
+  for (; v < v1; v += nt) {
+    uint4 q0 = xv[v];
+    V e[N];
+    Vec16<V>::unpack(q0, e);
+#pragma unroll
+    for (int j = 0; j < N; ++j) f(e[j], v * N + j);
+  }
+  uint32_t tail0 = (v1 * N > t0) ? v1 * N : t0;
+  for (uint32_t i = tail0 + threadIdx.x; i < t1; i += nt) f(x[i], i);
+}
+
+// Caller vectors may start at any element; 16-byte loads need 16-byte
+// alignment, so an unaligned base takes the one-element-per-lane path.
+template <class V, class F>
+__device__ __forceinline__ void stream_tile_any(const V* __restrict__ x, uint32_t t0, uint32_t t1, F&& f) {
+  if ((reinterpret_cast<uintptr_t>(x) & 15u) == 0) {
+    stream_tile<V>(x, t0, t1, f);
+  } else {
+    for (uint32_t i = t0 + threadIdx.x; i < t1; i += blockDim.x) f(x[i], i);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// pass-0 range table in LDS: sorted range starts + map parameters
+// ---------------------------------------------------------------------------
+template <class K> struct LdsRanges {
+  K* lo;
+  uint32_t* base;
+  uint32_t* sh;
+  uint32_t* mult;
+  uint32_t* nbm1;
+  uint32_t R, Rp;
+
+  __device__ static size_t bytes(uint32_t Rp) { return (size_t)Rp * (sizeof(K) + 16); }
+  __device__ void carve(unsigned char* p, uint32_t Rp_) {
+    Rp = Rp_;
+    lo = reinterpret_cast<K*>(p);
+    base = reinterpret_cast<uint32_t*>(p + (size_t)Rp * sizeof(K));
+    sh = base + Rp;
+    mult = sh + Rp;
+    nbm1 = mult + Rp;
+  }
+  __device__ void load(const Range<K>* __restrict__ g, uint32_t R_) {
+    R = R_;
+    for (uint32_t i = threadIdx.x; i < Rp; i += blockDim.x) {
+      if (i < R) {
+        Range<K> r = g[i];
+        lo[i] = r.lo; base[i] = r.base; sh[i] = r.sh; mult[i] = r.mult; nbm1[i] = r.nb - 1;
+      } else {
+        lo[i] = KeyMax<K>::v;
+        base[i] = 0; sh[i] = 0; mult[i] = 0; nbm1[i] = 0;
+      }
+    }
+  }
+  __device__ __forceinline__ uint32_t bucket(K key) const {
+    uint32_t r = 0;
+    for (uint32_t step = Rp >> 1; step; step >>= 1) {
+      uint32_t c = r + step;
+      if (lo[c] <= key) r = c;
+    }
+    r = r < R ? r : R - 1;
+    uint32_t off = map_off<K>(key, lo[r], sh[r], mult[r]);
+    off = off < nbm1[r] ? off : nbm1[r];
+    return base[r] + off;
+  }
+};
+
+// Plain fill (used instead of hipMemsetAsync so the sequence stays capturable).
+__global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t words, uint32_t v) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < words) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// k_init_ks: k-th largest (1-based) -> k-th smallest rank; validates range.
+// ---------------------------------------------------------------------------
+__global__ void k_init_ks(const uint32_t* __restrict__ ks, uint32_t nk, uint32_t n, uint32_t* __restrict__ k_rank,
+                          uint32_t* __restrict__ k_state, PassCtl* ctl0) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nk) return;
+  uint32_t k = ks[i];
+  if (k < 1 || k > n) {
+    atomicAdd(&ctl0->bad_k, 1u);
+    k_rank[i] = 1;
+    k_state[i] = kDone;
+    return;
+  }
+  k_rank[i] = n - k + 1;
+  k_state[i] = kLive;
+}
+
+// ---------------------------------------------------------------------------
+// k_count0: first pass over the caller's vector (values of type T).
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ void __launch_bounds__(1024) k_count0(const T* __restrict__ x, const Range<typename KeyTraits<T>::K>* __restrict__ ranges,
+                                                 const PassCtl* __restrict__ ctl, uint32_t* __restrict__ rows) {
+  using K = typename KeyTraits<T>::K;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const uint32_t B = ctl->B, n = ctl->n, tile = ctl->tile;
+  if (blockIdx.x >= ctl->G) return;
+  uint32_t* hist = reinterpret_cast<uint32_t*>(smem);
+  LdsRanges<K> rt;
+  rt.carve(smem + ((size_t)B * 4 + 15) / 16 * 16, ctl->Rp);
+  for (uint32_t i = threadIdx.x; i < B; i += blockDim.x) hist[i] = 0;
+  rt.load(ranges, ctl->R);
+  __syncthreads();
+  uint32_t t0 = blockIdx.x * tile;
+  uint32_t t1 = t0 + tile < n ? t0 + tile : n;
+  stream_tile_any<T>(x, t0, t1, [&](T v, uint32_t) {
+    uint32_t b = rt.bucket(KeyTraits<T>::enc(v));
+    lds_count_peeled(hist, b, true);
+  });
+  __syncthreads();
+  uint32_t* row = rows + (size_t)blockIdx.x * B;
+  for (uint32_t i = threadIdx.x; i < B; i += blockDim.x) row[i] = hist[i];
+}
+
+// ---------------------------------------------------------------------------
+// Segmented passes: the input buffer holds keys grouped by range; a range's
+// elements occupy [seg_start, seg_end).  Each lane follows the ranges along
+// its (increasing) positions, starting from the range that owns the
+// workgroup's first element.
+// ---------------------------------------------------------------------------
+template <class K> struct SegCursor {
+  const Range<K>* __restrict__ rg;
+  uint32_t r;
+  uint32_t seg_end;
+  K lo;
+  uint32_t base, sh, mult, nbm1;
+  __device__ __forceinline__ void load(uint32_t r_) {
+    r = r_;
+    Range<K> q = rg[r];
+    seg_end = q.seg_end; lo = q.lo; base = q.base; sh = q.sh; mult = q.mult; nbm1 = q.nb - 1;
+  }
+  __device__ __forceinline__ uint32_t bucket(K key, uint32_t pos) {
+    while (pos >= seg_end) load(r + 1);
+    uint32_t off = map_off<K>(key, lo, sh, mult);
+    off = off < nbm1 ? off : nbm1;
+    return base + off;
+  }
+};
+
+template <class K>
+__global__ void __launch_bounds__(1024) k_countS(const K* __restrict__ buf, const Range<K>* __restrict__ ranges,
+                                                 const uint32_t* __restrict__ wg_first, const PassCtl* __restrict__ ctl,
+                                                 uint32_t* __restrict__ rows) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const uint32_t B = ctl->B, n = ctl->n, tile = ctl->tile;
+  if (blockIdx.x >= ctl->G) return;
+  uint32_t* hist = reinterpret_cast<uint32_t*>(smem);
+  for (uint32_t i = threadIdx.x; i < B; i += blockDim.x) hist[i] = 0;
+  __syncthreads();
+  uint32_t t0 = blockIdx.x * tile;
+  uint32_t t1 = t0 + tile < n ? t0 + tile : n;
+  SegCursor<K> cur;
+  cur.rg = ranges;
+  cur.load(wg_first[blockIdx.x]);
+  stream_tile<K>(buf, t0, t1, [&](K key, uint32_t pos) {
+    uint32_t b = cur.bucket(key, pos);
+    lds_count_peeled(hist, b, true);
+  });
+  __syncthreads();
+  uint32_t* row = rows + (size_t)blockIdx.x * B;
+  for (uint32_t i = threadIdx.x; i < B; i += blockDim.x) row[i] = hist[i];
+}
+
+// ---------------------------------------------------------------------------
+// k_rowscan: rows[G][B] -> exclusive prefix down each column (in place) and
+// the column totals.  64 columns per workgroup, waves split the rows.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(1024) k_rowscan(uint32_t* __restrict__ rows, const PassCtl* __restrict__ ctl,
+                                                  uint32_t* __restrict__ totals) {
+  __shared__ uint32_t wsum[16][64];
+  const uint32_t G = ctl->G, B = ctl->B;
+  const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const uint32_t col = blockIdx.x * 64 + lane;
+  if (blockIdx.x * 64 >= B) return;
+  uint32_t rpw = (G + nw - 1) / nw;
+  uint32_t r0 = wv * rpw, r1 = r0 + rpw < G ? r0 + rpw : G;
+  uint32_t s = 0;
+  if (col < B)
+    for (uint32_t r = r0; r < r1; ++r) s += rows[(size_t)r * B + col];
+  wsum[wv][lane] = s;
+  __syncthreads();
+  uint32_t off = 0;
+  for (uint32_t j = 0; j < wv; ++j) off += wsum[j][lane];
+  if (wv == nw - 1 && col < B) totals[col] = off + s;
+  if (col < B)
+    for (uint32_t r = r0; r < r1; ++r) {
+      size_t idx = (size_t)r * B + col;
+      uint32_t v = rows[idx];
+      rows[idx] = off;
+      off += v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_plan: device-side findKBuckets + updatekVals + next-pass splitter.
+// ---------------------------------------------------------------------------
+template <class K> struct PlanArgs {
+  const uint32_t* totals;
+  const Range<K>* ranges;    // this pass (sorted by base)
+  const PassCtl* ctl;        // this pass
+  PassCtl* ctl_next;
+  Range<K>* next_ranges;
+  uint32_t nk;
+  uint32_t* k_rank;
+  uint32_t* k_state;
+  uint32_t* k_aux;
+  uint16_t* slot_of_bucket;  // [B]
+  uint32_t* slot_dst;        // [slots]
+  uint32_t* slot_bucket;     // [slots]
+  uint32_t* act_bucket;      // [U]
+  uint32_t* act_cls;
+  uint32_t* act_dst;
+  uint32_t* act_slot;
+  K* act_klo;
+  K* act_khi;
+  uint32_t* small_start;     // [U_small]
+  uint32_t* small_len;
+  uint32_t* small_kstart;    // [U_small + 1]
+  uint32_t* small_kfill;     // [U_small]
+  uint32_t* klist;           // [nk]
+  uint32_t* wg_first_next;   // [G_max]
+  void* out;                 // T[nk]
+  uint32_t fin_cap;          // largest segment the finish kernel sorts
+  uint32_t B_req;            // bucket budget of the next pass
+  uint32_t B_cap;            // hard cap (LDS)
+  uint32_t G_max;            // workgroups of the streaming kernels
+  uint32_t min_tile;         // smallest tile worth a workgroup
+  uint32_t force_finish;     // nonzero: stop after this pass (all active buckets must fit fin_cap)
+};
+
+template <class T>
+__global__ void __launch_bounds__(1024) k_plan(PlanArgs<typename KeyTraits<T>::K> a) {
+  using K = typename KeyTraits<T>::K;
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ uint32_t scratch[32];
+  __shared__ uint32_t sh_tot[8];
+  const uint32_t nt = blockDim.x, t = threadIdx.x;
+  const PassCtl& ctl = *a.ctl;
+  const uint32_t B = ctl.B;
+  uint32_t* P = reinterpret_cast<uint32_t*>(smem);     // inclusive prefix of the histogram
+  uint32_t* aidx = P + B;                              // active flag -> index + 1
+
+  // P0: histogram prefix, reset tables
+  for (uint32_t i = t; i < B; i += nt) {
+    P[i] = a.totals[i];
+    aidx[i] = 0;
+    a.slot_of_bucket[i] = (uint16_t)kNone16;
+  }
+  __syncthreads();
+  block_incl_scan_lds(P, B, scratch);
+
+  // P1: findKBuckets for every live k
+  for (uint32_t k = t; k < a.nk; k += nt) {
+    if (a.k_state[k] != kLive) continue;
+    uint32_t g = a.k_rank[k];
+    uint32_t lo = 0, hi = B - 1;  // first b with P[b] >= g
+    while (lo < hi) {
+      uint32_t mid = (lo + hi) >> 1;
+      if (P[mid] >= g) hi = mid; else lo = mid + 1;
+    }
+    aidx[lo] = 1;
+    a.k_aux[k] = lo;
+  }
+  __syncthreads();
+
+  // P2: number the active buckets in key order
+  {
+    uint32_t per = (B + nt - 1) / nt;
+    uint32_t b0 = t * per, b1 = b0 + per < B ? b0 + per : B;
+    uint32_t s = 0;
+    for (uint32_t i = b0; i < b1; ++i) s += aidx[i];
+    uint32_t tot;
+    uint32_t off = block_excl_scan_u32(s, scratch, &tot);
+    for (uint32_t i = b0; i < b1; ++i)
+      if (aidx[i]) aidx[i] = ++off;
+    if (t == 0) sh_tot[0] = tot;
+    __syncthreads();
+  }
+  const uint32_t U = sh_tot[0];
+
+  // P3: classify active buckets
+  for (uint32_t b = t; b < B; b += nt) {
+    uint32_t ai = aidx[b];
+    if (!ai) continue;
+    uint32_t q = ai - 1;
+    uint32_t c = a.totals[b];
+    // owning range: last r with base <= b
+    uint32_t lo = 0, hi = ctl.R - 1;
+    while (lo < hi) {
+      uint32_t mid = (lo + hi + 1) >> 1;
+      if (a.ranges[mid].base <= b) lo = mid; else hi = mid - 1;
+    }
+    Range<K> rg = a.ranges[lo];
+    K klo = 0, khi = 0;
+    bucket_interval<K>(rg, b - rg.base, klo, khi);
+    uint32_t cls;
+    if (klo == khi) cls = kResolved;
+    else if (c <= a.fin_cap) cls = kClsSmall;
+    else cls = a.force_finish ? kClsSmall : kClsBig;
+    a.act_bucket[q] = b;
+    a.act_cls[q] = cls;
+    a.act_klo[q] = klo;
+    a.act_khi[q] = khi;
+  }
+  __threadfence_block();
+  __syncthreads();
+
+  // P4: layout of the next buffer: big segments first, then small ones
+  {
+    uint32_t per = (U + nt - 1) / nt;
+    uint32_t q0 = t * per, q1 = q0 + per < U ? q0 + per : U;
+    uint32_t nbig = 0, cbig = 0, nsml = 0, csml = 0;
+    for (uint32_t q = q0; q < q1; ++q) {
+      uint32_t cls = a.act_cls[q], c = a.totals[a.act_bucket[q]];
+      if (cls == kClsBig) { nbig++; cbig += c; }
+      else if (cls == kClsSmall) { nsml++; csml += c; }
+    }
+    uint32_t T0, T1, T2, T3;
+    uint32_t e0 = block_excl_scan_u32(nbig, scratch, &T0);
+    uint32_t e1 = block_excl_scan_u32(cbig, scratch, &T1);
+    uint32_t e2 = block_excl_scan_u32(nsml, scratch, &T2);
+    uint32_t e3 = block_excl_scan_u32(csml, scratch, &T3);
+    for (uint32_t q = q0; q < q1; ++q) {
+      uint32_t cls = a.act_cls[q], c = a.totals[a.act_bucket[q]];
+      if (cls == kClsBig) {
+        a.act_slot[q] = e0++;
+        a.act_dst[q] = e1;
+        e1 += c;
+      } else if (cls == kClsSmall) {
+        a.act_slot[q] = T0 + e2++;
+        a.act_dst[q] = T1 + e3;
+        e3 += c;
+      } else {
+        a.act_slot[q] = kNone32;
+        a.act_dst[q] = 0;
+      }
+    }
+    if (t == 0) { sh_tot[1] = T0; sh_tot[2] = T1; sh_tot[3] = T2; sh_tot[4] = T3; }
+  }
+  __threadfence_block();
+  __syncthreads();
+  const uint32_t U_big = sh_tot[1], N_big = sh_tot[2], U_small = sh_tot[3], N_small = sh_tot[4];
+
+  // next-pass bucket budget: at least 2 buckets per big range when possible
+  uint32_t Bn = a.B_req;
+  if (Bn < 4 * U_big) Bn = 4 * U_big;
+  if (Bn > a.B_cap) Bn = a.B_cap;
+
+  // P5: slot tables, small segment descriptors, next ranges (unplaced)
+  for (uint32_t q = t; q < U; q += nt) {
+    uint32_t s = a.act_slot[q];
+    if (s == kNone32) continue;
+    uint32_t b = a.act_bucket[q];
+    uint32_t c = a.totals[b];
+    uint32_t dst = a.act_dst[q];
+    a.slot_of_bucket[b] = (uint16_t)s;
+    a.slot_dst[s] = dst;
+    a.slot_bucket[s] = b;
+    if (s < U_big) {
+      Range<K> nr;
+      nr.lo = a.act_klo[q];
+      nr.hi = a.act_khi[q];
+      nr.seg_start = dst;
+      nr.seg_end = dst + c;
+      nr.base = 0;
+      uint32_t nb;
+      if (2 * U_big <= Bn) nb = 2 + (uint32_t)(((uint64_t)c * (Bn - 2 * U_big)) / N_big);
+      else nb = 1 + (uint32_t)(((uint64_t)c * (Bn - U_big)) / N_big);
+      make_map<K>(nr, nb);
+      a.next_ranges[s] = nr;
+    } else {
+      uint32_t j = s - U_big;
+      a.small_start[j] = dst;
+      a.small_len[j] = c;
+      a.small_kfill[j] = 0;
+    }
+  }
+  __threadfence_block();
+  __syncthreads();
+
+  // P6: bucket bases of the next pass
+  uint32_t B_next = 0;
+  {
+    uint32_t per = (U_big + nt - 1) / nt;
+    uint32_t j0 = t * per, j1 = j0 + per < U_big ? j0 + per : U_big;
+    uint32_t s = 0;
+    for (uint32_t j = j0; j < j1; ++j) s += a.next_ranges[j].nb;
+    uint32_t tot;
+    uint32_t off = block_excl_scan_u32(s, scratch, &tot);
+    for (uint32_t j = j0; j < j1; ++j) {
+      a.next_ranges[j].base = off;
+      off += a.next_ranges[j].nb;
+    }
+    B_next = tot;
+  }
+
+  // P7: re-rank live ks
+  T* out = reinterpret_cast<T*>(a.out);
+  for (uint32_t k = t; k < a.nk; k += nt) {
+    if (a.k_state[k] != kLive) continue;
+    uint32_t b = a.k_aux[k];
+    uint32_t q = aidx[b] - 1;
+    uint32_t c = a.totals[b];
+    uint32_t local = a.k_rank[k] - (P[b] - c);
+    uint32_t cls = a.act_cls[q];
+    if (cls == kResolved) {
+      out[k] = KeyTraits<T>::dec(a.act_klo[q]);
+      a.k_state[k] = kDone;
+    } else if (cls == kClsBig) {
+      a.k_rank[k] = a.act_dst[q] + local;
+    } else {
+      uint32_t j = a.act_slot[q] - U_big;
+      a.k_rank[k] = local;
+      a.k_aux[k] = j;
+      a.k_state[k] = kSmallNew;
+      atomicAdd(&a.small_kfill[j], 1u);
+    }
+  }
+  __threadfence_block();
+  __syncthreads();
+
+  // P8: k lists of the small segments (counting sort by segment)
+  {
+    uint32_t per = (U_small + nt - 1) / nt;
+    uint32_t j0 = t * per, j1 = j0 + per < U_small ? j0 + per : U_small;
+    uint32_t s = 0;
+    for (uint32_t j = j0; j < j1; ++j) s += a.small_kfill[j];
+    uint32_t tot;
+    uint32_t off = block_excl_scan_u32(s, scratch, &tot);
+    for (uint32_t j = j0; j < j1; ++j) {
+      a.small_kstart[j] = off;
+      off += a.small_kfill[j];
+      a.small_kfill[j] = 0;
+    }
+    if (t == 0) a.small_kstart[U_small] = tot;
+  }
+  __threadfence_block();
+  __syncthreads();
+  uint32_t live = 0;
+  for (uint32_t k = t; k < a.nk; k += nt) {
+    uint32_t st = a.k_state[k];
+    if (st == kSmallNew) {
+      uint32_t j = a.k_aux[k];
+      uint32_t pos = atomicAdd(&a.small_kfill[j], 1u);
+      a.klist[a.small_kstart[j] + pos] = k;
+      a.k_state[k] = kSmall;
+    }
+    if (st == kLive) live++;
+  }
+  uint32_t live_tot;
+  block_excl_scan_u32(live, scratch, &live_tot);
+
+  // P9: next-pass tiling and first range of every tile
+  uint32_t Gn = 0, tile = 0;
+  if (N_big > 0) {
+    Gn = (N_big + a.min_tile - 1) / a.min_tile;
+    if (Gn > a.G_max) Gn = a.G_max;
+    if (Gn < 1) Gn = 1;
+    tile = (N_big + Gn - 1) / Gn;
+    tile = (tile + 63) & ~63u;
+    Gn = (N_big + tile - 1) / tile;
+  }
+  for (uint32_t j = t; j < U_big; j += nt) {
+    Range<K> r = a.next_ranges[j];
+    uint32_t w0 = (r.seg_start + tile - 1) / tile;
+    uint32_t w1 = (r.seg_end + tile - 1) / tile;
+    for (uint32_t w = w0; w < w1 && w < Gn; ++w) a.wg_first_next[w] = j;
+  }
+  if (t == 0) {
+    PassCtl* cn = a.ctl_next;
+    cn->n = N_big;
+    cn->tile = tile;
+    cn->G = Gn;
+    cn->R = U_big;
+    cn->Rp = pow2ceil_u32(U_big);
+    cn->B = B_next;
+    cn->nan_lo_bucket = kNone32;
+    cn->nan_hi_bucket = kNone32;
+    PassCtl* cc = const_cast<PassCtl*>(a.ctl);
+    cc->U = U;
+    cc->U_big = U_big;
+    cc->U_small = U_small;
+    cc->U_slots = U_big + U_small;
+    cc->N_big = N_big;
+    cc->N_small = N_small;
+    cc->live_ks = live_tot;
+    uint32_t nan = 0;
+    if (ctl.nan_lo_bucket != kNone32) nan += a.totals[ctl.nan_lo_bucket];
+    if (ctl.nan_hi_bucket != kNone32) nan += a.totals[ctl.nan_hi_bucket];
+    cc->nan_count = nan;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Compaction: copy every element of an active (small or big) bucket to its
+// segment.  Workgroup w starts each segment cursor at
+//   slot_dst[s] + (elements of that bucket in workgroups < w)
+// so the segments come out contiguous without any global atomics.
+// ---------------------------------------------------------------------------
+struct CompactTables {
+  uint16_t* slot;    // [B]
+  uint32_t* cursor;  // [U_slots]
+};
+
+__device__ __forceinline__ void compact_init(CompactTables& ct, unsigned char* p, uint32_t B, uint32_t U_slots,
+                                             const uint16_t* __restrict__ slot_of_bucket,
+                                             const uint32_t* __restrict__ slot_dst,
+                                             const uint32_t* __restrict__ slot_bucket,
+                                             const uint32_t* __restrict__ row_prefix) {
+  ct.slot = reinterpret_cast<uint16_t*>(p);
+  ct.cursor = reinterpret_cast<uint32_t*>(p + ((size_t)B * 2 + 15) / 16 * 16);
+  for (uint32_t i = threadIdx.x; i < B; i += blockDim.x) ct.slot[i] = slot_of_bucket[i];
+  for (uint32_t s = threadIdx.x; s < U_slots; s += blockDim.x)
+    ct.cursor[s] = slot_dst[s] + row_prefix[slot_bucket[s]];
+}
+
+__device__ __forceinline__ size_t compact_lds_bytes(uint32_t B, uint32_t U_slots) {
+  return ((size_t)B * 2 + 15) / 16 * 16 + (size_t)U_slots * 4;
+}
+
+template <class T>
+__global__ void __launch_bounds__(1024) k_compact0(const T* __restrict__ x, const Range<typename KeyTraits<T>::K>* __restrict__ ranges,
+                                                   const PassCtl* __restrict__ ctl, const uint32_t* __restrict__ rows,
+                                                   const uint16_t* __restrict__ slot_of_bucket,
+                                                   const uint32_t* __restrict__ slot_dst,
+                                                   const uint32_t* __restrict__ slot_bucket,
+                                                   typename KeyTraits<T>::K* __restrict__ out) {
+  using K = typename KeyTraits<T>::K;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const uint32_t B = ctl->B, n = ctl->n, tile = ctl->tile, U_slots = ctl->U_slots;
+  if (blockIdx.x >= ctl->G || U_slots == 0) return;
+  CompactTables ct;
+  compact_init(ct, smem, B, U_slots, slot_of_bucket, slot_dst, slot_bucket, rows + (size_t)blockIdx.x * B);
+  LdsRanges<K> rt;
+  rt.carve(smem + compact_lds_bytes(B, U_slots + 4), ctl->Rp);
+  rt.load(ranges, ctl->R);
+  __syncthreads();
+  uint32_t t0 = blockIdx.x * tile;
+  uint32_t t1 = t0 + tile < n ? t0 + tile : n;
+  stream_tile_any<T>(x, t0, t1, [&](T v, uint32_t) {
+    K key = KeyTraits<T>::enc(v);
+    uint32_t s = ct.slot[rt.bucket(key)];
+    bool act = s != kNone16;
+    uint32_t pos = lds_claim_peeled(ct.cursor, s, act);
+    if (act) out[pos] = key;
+  });
+}
+
+template <class K>
+__global__ void __launch_bounds__(1024) k_compactS(const K* __restrict__ buf, const Range<K>* __restrict__ ranges,
+                                                   const uint32_t* __restrict__ wg_first, const PassCtl* __restrict__ ctl,
+                                                   const uint32_t* __restrict__ rows,
+                                                   const uint16_t* __restrict__ slot_of_bucket,
+                                                   const uint32_t* __restrict__ slot_dst,
+                                                   const uint32_t* __restrict__ slot_bucket, K* __restrict__ out) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const uint32_t B = ctl->B, n = ctl->n, tile = ctl->tile, U_slots = ctl->U_slots;
+  if (blockIdx.x >= ctl->G || U_slots == 0) return;
+  CompactTables ct;
+  compact_init(ct, smem, B, U_slots, slot_of_bucket, slot_dst, slot_bucket, rows + (size_t)blockIdx.x * B);
+  __syncthreads();
+  uint32_t t0 = blockIdx.x * tile;
+  uint32_t t1 = t0 + tile < n ? t0 + tile : n;
+  SegCursor<K> cur;
+  cur.rg = ranges;
+  cur.load(wg_first[blockIdx.x]);
+  stream_tile<K>(buf, t0, t1, [&](K key, uint32_t pos) {
+    uint32_t s = ct.slot[cur.bucket(key, pos)];
+    bool act = s != kNone16;
+    uint32_t p = lds_claim_peeled(ct.cursor, s, act);
+    if (act) out[p] = key;
+  });
+}
+
+// ---------------------------------------------------------------------------
+// k_finish: exact answers for the ks of every small segment.
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ void __launch_bounds__(1024) k_finish(const typename KeyTraits<T>::K* __restrict__ buf, const PassCtl* __restrict__ ctl,
+                                                 const uint32_t* __restrict__ small_start,
+                                                 const uint32_t* __restrict__ small_len,
+                                                 const uint32_t* __restrict__ small_kstart,
+                                                 const uint32_t* __restrict__ klist, uint32_t* __restrict__ k_rank,
+                                                 uint32_t* __restrict__ k_state, T* __restrict__ out,
+                                                 uint32_t fin_cap, uint32_t* __restrict__ err) {
+  using K = typename KeyTraits<T>::K;
+  extern __shared__ __align__(16) unsigned char smem[];
+  K* s = reinterpret_cast<K*>(smem);
+  const uint32_t U_small = ctl->U_small;
+  for (uint32_t j = blockIdx.x; j < U_small; j += gridDim.x) {
+    uint32_t st = small_start[j], len = small_len[j];
+    if (len > fin_cap) {  // cannot happen unless the pass cap forced finishing; reported, never sorted
+      if (threadIdx.x == 0) atomicAdd(err, 1u);
+      continue;
+    }
+    uint32_t M = pow2ceil_u32(len);
+    for (uint32_t i = threadIdx.x; i < M; i += blockDim.x) s[i] = i < len ? buf[st + i] : KeyMax<K>::v;
+    __syncthreads();
+    bitonic_sort_lds<K>(s, M);
+    for (uint32_t q = small_kstart[j] + threadIdx.x; q < small_kstart[j + 1]; q += blockDim.x) {
+      uint32_t k = klist[q];
+      uint32_t r = k_rank[k] - 1;
+      out[k] = KeyTraits<T>::dec(s[r]);
+      k_state[k] = kDone;
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Tiny inputs (n <= finish capacity): one segment holding everything.
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ void k_tiny_setup(const T* __restrict__ x, uint32_t n, uint32_t nk, typename KeyTraits<T>::K* __restrict__ buf,
+                             uint32_t* __restrict__ k_rank, uint32_t* __restrict__ k_state,
+                             uint32_t* __restrict__ small_start, uint32_t* __restrict__ small_len,
+                             uint32_t* __restrict__ small_kstart, uint32_t* __restrict__ klist, PassCtl* ctl) {
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) buf[i] = KeyTraits<T>::enc(x[i]);
+  __shared__ uint32_t scratch[32];
+  uint32_t cnt = 0;
+  for (uint32_t k = threadIdx.x; k < nk; k += blockDim.x) cnt += (k_state[k] == kLive);
+  uint32_t tot;
+  uint32_t off = block_excl_scan_u32(cnt, scratch, &tot);
+  for (uint32_t k = threadIdx.x; k < nk; k += blockDim.x)
+    if (k_state[k] == kLive) {
+      klist[off++] = k;
+      k_state[k] = kSmall;
+    }
+  if (threadIdx.x == 0) {
+    small_start[0] = 0;
+    small_len[0] = n;
+    small_kstart[0] = 0;
+    small_kstart[1] = tot;
+    ctl->U_small = 1;
+    ctl->U_big = 0;
+    ctl->N_big = 0;
+    ctl->U = 1;
+  }
+}
+
+}  // namespace ggms

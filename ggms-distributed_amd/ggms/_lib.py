@@ -1,0 +1,112 @@
+"""ctypes binding of libggms.so (the C ABI declared in include/ggms.h).
+
+The library must have been built (ggms-distributed_amd/build.py or
+__graft_entry__.build()); loading fails loudly otherwise -- there is no CPU
+fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libggms.so")
+
+GGMS_MAX_PASS_STATS = 64
+KERNEL_KINDS = ["plan0", "count0", "countS", "rowscan", "plan", "compact0", "compactS", "finish", "other"]
+
+
+class Opts(C.Structure):
+    _fields_ = [
+        ("num_buckets", C.c_uint32),
+        ("num_pivots", C.c_uint32),
+        ("sample_size", C.c_uint32),
+        ("block_threads", C.c_uint32),
+        ("blocks_per_cu", C.c_uint32),
+        ("splitter", C.c_uint32),
+        ("learned_pieces", C.c_uint32),
+        ("max_passes", C.c_uint32),
+        ("seed", C.c_uint64),
+        ("stream", C.c_void_p),
+        ("profile", C.c_uint32),
+        ("reserved", C.c_uint32 * 7),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("passes", C.c_uint32),
+        ("finished_segments", C.c_uint32),
+        ("n_in", C.c_uint64 * GGMS_MAX_PASS_STATS),
+        ("active", C.c_uint32 * GGMS_MAX_PASS_STATS),
+        ("n_kept", C.c_uint64 * GGMS_MAX_PASS_STATS),
+        ("buckets", C.c_uint32 * GGMS_MAX_PASS_STATS),
+        ("count_ms", C.c_float * GGMS_MAX_PASS_STATS),
+        ("compact_ms", C.c_float * GGMS_MAX_PASS_STATS),
+        ("kernel_ms", C.c_float * 9),
+        ("kernel_launches", C.c_uint32 * 9),
+    ]
+
+
+class RangeU64(C.Structure):
+    _fields_ = [
+        ("lo", C.c_uint64),
+        ("hi", C.c_uint64),
+        ("base", C.c_uint32),
+        ("nb", C.c_uint32),
+        ("sh", C.c_uint32),
+        ("mult", C.c_uint32),
+    ]
+
+
+# (name, restype, argtypes) of every symbol include/ggms.h declares.
+_P = C.c_void_p
+_U32P = C.POINTER(C.c_uint32)
+SIGNATURES = {
+    "ggms_opts_default": (None, [C.POINTER(Opts)]),
+    "ggms_status_str": (C.c_char_p, [C.c_int]),
+    "ggms_ctx_create": (C.c_int, [C.c_int, C.POINTER(_P)]),
+    "ggms_ctx_destroy": (None, [_P]),
+    "ggms_ctx_stats": (C.c_int, [_P, C.POINTER(Stats)]),
+    "ggms_workspace_bytes": (C.c_uint64, [C.c_int, C.c_uint64, C.c_uint32, C.POINTER(Opts)]),
+    "ggms_select": (C.c_int, [_P, C.c_int, C.c_int, _P, C.c_uint64, _U32P, C.c_uint32, _P, C.POINTER(Opts)]),
+    "ggms_select_device": (C.c_int, [_P, C.c_int, C.c_int, _P, C.c_uint64, _P, C.c_uint32, _P, C.POINTER(Opts)]),
+    "ggms_debug_first_pass": (C.c_int, [_P, C.c_int, _P, C.c_uint64, C.POINTER(Opts), C.POINTER(RangeU64), _U32P,
+                                        _U32P, _U32P]),
+}
+SIGNATURES["ggms_generate"] = (C.c_int, [C.c_int, C.c_int, _P, C.c_uint64, C.c_uint64, C.c_double, C.c_double, _P])
+SIGNATURES["ggms_mixture_params"] = (C.c_int, [C.c_uint64, C.POINTER(C.c_int), _P, _P])
+SIGNATURES["ggms_generate_ks"] = (C.c_int, [C.c_int, C.c_uint32, C.c_uint64, C.c_uint64, _U32P])
+for _alg in ("iterativeSMOSWrapper", "bucketMultiselectWrapper"):
+    for _suf in ("f32", "f64", "u32", "i32"):
+        SIGNATURES[f"ggms_{_alg}_{_suf}"] = (C.c_int, [_P, C.c_int, _U32P, C.c_int, _P, C.c_int, C.c_int])
+for _suf in ("f32", "f64", "u32", "i32"):
+    SIGNATURES[f"ggms_distributedSMOSWrapper_{_suf}"] = (C.c_int, [_P, C.c_int, _U32P, C.c_int, _P, C.c_int, C.c_int,
+                                                                  C.c_int])
+
+_lib = None
+
+
+def load(path: str = LIB_PATH) -> C.CDLL:
+    """Load libggms.so and attach prototypes.  Raises if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"libggms.so not built at {path}; run ggms-distributed_amd/build.py")
+    lib = C.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def extra_signature(name: str, restype, argtypes):
+    """Register a symbol added by a later module (generators, batched, MLP)."""
+    SIGNATURES[name] = (restype, argtypes)
+    if _lib is not None:
+        fn = getattr(_lib, name)
+        fn.restype = restype
+        fn.argtypes = argtypes

@@ -1,0 +1,206 @@
+/*
+ * ggms.h -- C ABI of the MI355X multiple-order-statistics selection library.
+ *
+ * Given a device vector of n values and a list of k ranks, return the k-th
+ * value for every rank.  Ranks follow the reference wrapper convention:
+ * k = 1 is the LARGEST value, k = n the smallest
+ * (reference: DistributedSMOS/iterativeSMOS.cu:1587-1588 converts k-th largest
+ * to k-th smallest with length - k + 1; the sort-and-choose oracle picks
+ * sorted[n - k], DistributedSMOS/iterativeSMOSTimingFunctions.cu:67-73).
+ *
+ * Results are bit-exact: values are compared through an order-preserving
+ * integer key (fp32 -> u32, fp64 -> u64), so -0.0 orders below +0.0.  An input
+ * holding a NaN is rejected with GGMS_ENAN.
+ *
+ * All pointers named d_* are device pointers (hipMalloc'd); *_host pointers are
+ * host memory.  No function keeps a pointer after it returns.  The input vector
+ * is never modified (the reference's iterativeSMOS clobbers it; this library
+ * keeps it intact).
+ */
+#ifndef GGMS_H
+#define GGMS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  GGMS_OK = 0,
+  GGMS_EINVAL = 1,   /* bad argument: null pointer, n == 0, k outside [1, n], unknown enum */
+  GGMS_ENAN = 2,     /* the input vector holds a NaN */
+  GGMS_ENOMEM = 3,   /* device or host allocation failed */
+  GGMS_EDEVICE = 4,  /* HIP runtime error (launch, copy, sync) */
+  GGMS_ENOTSUP = 5   /* configuration not supported (e.g. learned splitter without weights) */
+} ggms_status;
+
+typedef enum { GGMS_F32 = 0, GGMS_F64 = 1, GGMS_U32 = 2, GGMS_I32 = 3 } ggms_dtype;
+
+typedef enum {
+  GGMS_ALG_BUCKET = 0,          /* bucketMultiselect: one bucketing pass, then sort of the k-buckets */
+  GGMS_ALG_ITERATIVE = 1,       /* iterativeSMOS: re-bucket the k-buckets until they are small */
+  GGMS_ALG_DISTRIBUTED_1GPU = 2 /* distributedSMOS collapsed to one GPU: strided sample, no exchange */
+} ggms_alg;
+
+typedef enum {
+  GGMS_SPLIT_LINEAR = 0,  /* reference splitter: 1024 random samples, 17 pivots, linear slope per range */
+  GGMS_SPLIT_LEARNED = 1, /* MLP-predicted pivots from a sorted-sample sketch; linear slope inside ranges */
+  GGMS_SPLIT_SKETCH = 2   /* same sample and range count as LEARNED, pivots read straight off the sample */
+} ggms_splitter;
+
+/* Tunables.  Zero means "library default" for every numeric field. */
+typedef struct {
+  uint32_t num_buckets;   /* buckets per pass (default 8192, max 16384) */
+  uint32_t num_pivots;    /* linear splitter pivots (default 17 => 16 ranges) */
+  uint32_t sample_size;   /* splitter sample (default 1024; 8192 for the one-GPU distributed form) */
+  uint32_t block_threads; /* workgroup size of the streaming kernels (default 1024) */
+  uint32_t blocks_per_cu; /* streaming workgroups per CU (default 2) */
+  uint32_t splitter;      /* ggms_splitter (default LINEAR) */
+  uint32_t learned_pieces;/* ranges produced by the learned / sketch splitter (default 256) */
+  uint32_t max_passes;    /* safety cap on bucketing passes (default 64) */
+  uint64_t seed;          /* sampling seed (default 0x5EED) */
+  void* stream;           /* hipStream_t to run on; NULL = the null stream */
+  uint32_t profile;       /* nonzero: time every kernel with HIP events on the launch stream (see ggms_stats) */
+  uint32_t reserved[7];
+} ggms_opts;
+
+/* Per-call statistics of the last selection run on a context. */
+#define GGMS_MAX_PASS_STATS 64
+/* kernel kinds timed when ggms_opts.profile is set */
+enum {
+  GGMS_KERNEL_PLAN0 = 0, GGMS_KERNEL_COUNT0 = 1, GGMS_KERNEL_COUNTS = 2, GGMS_KERNEL_ROWSCAN = 3, GGMS_KERNEL_PLAN = 4,
+  GGMS_KERNEL_COMPACT0 = 5, GGMS_KERNEL_COMPACTS = 6, GGMS_KERNEL_FINISH = 7, GGMS_KERNEL_OTHER = 8,
+  GGMS_KERNEL_KINDS = 9
+};
+typedef struct {
+  uint32_t passes;                          /* bucketing passes run */
+  uint32_t finished_segments;               /* k-buckets resolved by the LDS sort */
+  uint64_t n_in[GGMS_MAX_PASS_STATS];       /* elements streamed by the count kernel of each pass */
+  uint32_t active[GGMS_MAX_PASS_STATS];     /* active (k-holding) buckets found by each pass */
+  uint64_t n_kept[GGMS_MAX_PASS_STATS];     /* elements copied forward by each pass's compaction */
+  uint32_t buckets[GGMS_MAX_PASS_STATS];    /* buckets used by each pass */
+  float count_ms[GGMS_MAX_PASS_STATS];      /* profile: count kernel time of each pass */
+  float compact_ms[GGMS_MAX_PASS_STATS];    /* profile: compaction kernel time of each pass */
+  float kernel_ms[GGMS_KERNEL_KINDS];       /* profile: total time per kernel kind */
+  uint32_t kernel_launches[GGMS_KERNEL_KINDS];
+} ggms_stats;
+
+typedef struct ggms_ctx ggms_ctx;
+
+/* Fill *opts with library defaults. */
+void ggms_opts_default(ggms_opts* opts);
+
+/* Human-readable status name. */
+const char* ggms_status_str(ggms_status s);
+
+/* A context owns the device workspace (grown on demand, reused across calls)
+ * for one device.  A context is not re-entrant: use one per host thread. */
+ggms_status ggms_ctx_create(int device, ggms_ctx** out);
+void ggms_ctx_destroy(ggms_ctx* ctx);
+ggms_status ggms_ctx_stats(const ggms_ctx* ctx, ggms_stats* out);
+
+/* Device workspace bytes a call with these arguments needs. */
+uint64_t ggms_workspace_bytes(ggms_dtype t, uint64_t n, uint32_t nk, const ggms_opts* opts);
+
+/*
+ * Multiselect.  d_vec: n values of type t on the device.  ks_host: nk ranks in
+ * [1, n] (k = 1 is the largest), any order, duplicates allowed.  out_host: nk
+ * values written in the order of ks_host.  opts may be NULL.
+ * Replaces IterativeSMOS::iterativeSMOSWrapper (DistributedSMOS/iterativeSMOS.cuh:218-220),
+ * BucketMultiselect::bucketMultiselectWrapper (DistributedSMOS/bucketMultiselect.cuh:105-107)
+ * and DistributedSMOS::distributedSMOSWrapper (DistributedSMOS/distributedSMOS.hpp:22-25)
+ * through the alg argument.
+ */
+ggms_status ggms_select(ggms_ctx* ctx, ggms_alg alg, ggms_dtype t, const void* d_vec, uint64_t n,
+                        const uint32_t* ks_host, uint32_t nk, void* out_host, const ggms_opts* opts);
+
+/* Same, with ks and outputs in device memory; asynchronous with respect to the
+ * host except for one small status read per bucketing pass. */
+ggms_status ggms_select_device(ggms_ctx* ctx, ggms_alg alg, ggms_dtype t, const void* d_vec, uint64_t n,
+                               const uint32_t* d_ks, uint32_t nk, void* d_out, const ggms_opts* opts);
+
+/*
+ * Reference call shapes, one per instantiated type, using the library's
+ * default context of the current device.  blocks / threads are accepted for
+ * signature compatibility (the reference passes SM count and max threads) and
+ * only threads is honoured, as the workgroup size when it is a multiple of 64.
+ * Return value: GGMS_OK or an error status (the reference returns T and
+ * ignores errors).
+ *   IterativeSMOS::iterativeSMOSWrapper   DistributedSMOS/iterativeSMOS.cu:1579-1612
+ *   BucketMultiselect::bucketMultiselectWrapper DistributedSMOS/bucketMultiselect.cu:852-899
+ *   DistributedSMOS::distributedSMOSWrapper DistributedSMOS/distributedSMOS.cpp:1208-1280
+ */
+int ggms_iterativeSMOSWrapper_f32(float* d_vector, int length, const uint32_t* kVals_ori, int numKs, float* outputs, int blocks, int threads);
+int ggms_iterativeSMOSWrapper_f64(double* d_vector, int length, const uint32_t* kVals_ori, int numKs, double* outputs, int blocks, int threads);
+int ggms_iterativeSMOSWrapper_u32(uint32_t* d_vector, int length, const uint32_t* kVals_ori, int numKs, uint32_t* outputs, int blocks, int threads);
+int ggms_iterativeSMOSWrapper_i32(int32_t* d_vector, int length, const uint32_t* kVals_ori, int numKs, int32_t* outputs, int blocks, int threads);
+int ggms_bucketMultiselectWrapper_f32(float* d_vector, int length, const uint32_t* kVals_ori, int numKs, float* outputs, int blocks, int threads);
+int ggms_bucketMultiselectWrapper_f64(double* d_vector, int length, const uint32_t* kVals_ori, int numKs, double* outputs, int blocks, int threads);
+int ggms_bucketMultiselectWrapper_u32(uint32_t* d_vector, int length, const uint32_t* kVals_ori, int numKs, uint32_t* outputs, int blocks, int threads);
+int ggms_bucketMultiselectWrapper_i32(int32_t* d_vector, int length, const uint32_t* kVals_ori, int numKs, int32_t* outputs, int blocks, int threads);
+int ggms_distributedSMOSWrapper_f32(float* d_vector, int length, const uint32_t* kVals_ori, int numKs, float* outputs, int blocks, int threads, int rank);
+int ggms_distributedSMOSWrapper_f64(double* d_vector, int length, const uint32_t* kVals_ori, int numKs, double* outputs, int blocks, int threads, int rank);
+int ggms_distributedSMOSWrapper_u32(uint32_t* d_vector, int length, const uint32_t* kVals_ori, int numKs, uint32_t* outputs, int blocks, int threads, int rank);
+int ggms_distributedSMOSWrapper_i32(int32_t* d_vector, int length, const uint32_t* kVals_ori, int numKs, int32_t* outputs, int blocks, int threads, int rank);
+
+/*
+ * Introspection of the first bucketing pass (used by the parity tests to check
+ * the splitter and the fused count kernel against the oracle): runs sampling,
+ * pivot/range construction and the count kernel, and copies the range table and
+ * the bucket histogram to the host.  ranges_host receives *num_ranges records of
+ * ggms_range_u64 (keys widened to 64 bits); hist_host receives *num_buckets
+ * counts.  Capacities are given in *num_ranges / *num_buckets on entry.
+ */
+typedef struct {
+  uint64_t lo, hi;  /* key interval, inclusive */
+  uint32_t base;    /* first bucket */
+  uint32_t nb;      /* bucket count */
+  uint32_t sh;      /* shift of (key - lo) */
+  uint32_t mult;    /* fixed-point slope; 0 = identity */
+} ggms_range_u64;
+
+ggms_status ggms_debug_first_pass(ggms_ctx* ctx, ggms_dtype t, const void* d_vec, uint64_t n, const ggms_opts* opts,
+                                  ggms_range_u64* ranges_host, uint32_t* num_ranges, uint32_t* hist_host,
+                                  uint32_t* num_buckets);
+
+
+/* ------------------------------------------------------------------------
+ * Seeded synthetic inputs (replace DistributedSMOS/generateProblems.cu:43-513,
+ * which draw from cuRAND seeded by wall-clock time).  Element i is a pure
+ * function of (seed, i) through Philox4x32-10; see generate.hip for the exact
+ * definitions.  p0 / p1 parametrise some distributions:
+ *   UNIFORM       u32/i32: p0 > 0 scales into [0, p0)
+ *   NEAR_CONSTANT value p0, except a fraction p1 of elements drawn NORMAL
+ *   CONSTANT      value p0
+ *   MOD_INDEX     i mod p0
+ * ------------------------------------------------------------------------ */
+typedef enum {
+  GGMS_DIST_UNIFORM = 0, GGMS_DIST_NORMAL = 1, GGMS_DIST_HALF_NORMAL = 2, GGMS_DIST_LOGNORMAL = 3,
+  GGMS_DIST_MIXTURE = 4, GGMS_DIST_NEAR_CONSTANT = 5, GGMS_DIST_CONSTANT = 6, GGMS_DIST_CAUCHY = 7,
+  GGMS_DIST_MOD_INDEX = 8, GGMS_DIST_ASCENDING = 9
+} ggms_dist;
+
+/* Fill d_out[0..n) on the device (asynchronous on stream). */
+ggms_status ggms_generate(ggms_dtype t, ggms_dist d, void* d_out, uint64_t n, uint64_t seed, double p0, double p1,
+                          void* stream);
+
+/* Parameters of the GGMS_DIST_MIXTURE components for a seed: m in [1, 8], mu[8], sigma[8]. */
+ggms_status ggms_mixture_params(uint64_t seed, int* m, double* mu, double* sigma);
+
+/* k-rank lists (host).  Reference k distributions of
+ * DistributedSMOS/generateProblems.cu:584-720 plus DISTINCT_SORTED (distinct
+ * uniform ranks, ascending), EVEN (ceil(i*n/nk)) and WINDOW (uniform inside a
+ * window of max(nk, n/100) ranks at a random offset). */
+typedef enum {
+  GGMS_K_UNIFORM_RANDOM = 0, GGMS_K_UNIFORM = 1, GGMS_K_NORMAL = 2, GGMS_K_CLUSTER = 3, GGMS_K_SECTIONED = 4,
+  GGMS_K_DISTINCT_SORTED = 5, GGMS_K_EVEN = 6, GGMS_K_WINDOW = 7
+} ggms_kdist;
+
+ggms_status ggms_generate_ks(ggms_kdist kd, uint32_t nk, uint64_t n, uint64_t seed, uint32_t* ks_host);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GGMS_H */
