@@ -154,35 +154,49 @@ __device__ __forceinline__ uint32_t readlane_u32(uint32_t v, uint32_t lane) {
   return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)lane);
 }
 
-// hist[b] += 1 for every active lane.  The bucket of the wave's first active
-// lane is peeled off with one ballot and one atomic, so runs of duplicates (or
-// one dominant bucket) cost a single LDS atomic instead of 64 serialised ones.
-__device__ __forceinline__ void lds_count_peeled(uint32_t* hist, uint32_t b, bool active) {
+// Duplicate peeling: when at least kPeelMin active lanes of a wave share the
+// first active lane's bucket, that bucket takes one atomic for all of them, so
+// a dominant value costs one LDS atomic instead of 64 serialised ones.  The
+// test is a single compare + ballot and the branch is wave-uniform, so random
+// data pays almost nothing for it.
+constexpr uint32_t kPeelMin = 16;
+
+// hist[b] += 1 for every active lane with `active` set.
+__device__ __forceinline__ void lds_count(uint32_t* hist, uint32_t b, bool active = true) {
   uint64_t am = wave_ballot(active);
   if (am == 0) return;
   uint32_t first = (uint32_t)__builtin_ctzll(am);
   uint32_t b0 = readlane_u32(b, first);
   bool same = active && (b == b0);
   uint64_t m0 = wave_ballot(same);
-  if (lane_id() == first) atomicAdd(&hist[b0], (uint32_t)__builtin_popcountll(m0));
-  if (active && !same) atomicAdd(&hist[b], 1u);
+  uint32_t c0 = (uint32_t)__builtin_popcountll(m0);
+  if (c0 >= kPeelMin) {
+    if (lane_id() == first) atomicAdd(&hist[b0], c0);
+    if (active && !same) atomicAdd(&hist[b], 1u);
+  } else if (active) {
+    atomicAdd(&hist[b], 1u);
+  }
 }
 
-// Returns this lane's slot position: pos = cursor[s]++ (LDS), peeled the same
-// way.  Only meaningful for active lanes.
-__device__ __forceinline__ uint32_t lds_claim_peeled(uint32_t* cursor, uint32_t s, bool active) {
+// pos = cursor[s]++ for every active lane (LDS), with the same peeling.
+__device__ __forceinline__ uint32_t lds_claim(uint32_t* cursor, uint32_t s, bool active) {
   uint64_t am = wave_ballot(active);
   if (am == 0) return 0;
   uint32_t first = (uint32_t)__builtin_ctzll(am);
   uint32_t s0 = readlane_u32(s, first);
   bool same = active && (s == s0);
   uint64_t m0 = wave_ballot(same);
-  uint32_t base0 = 0;
-  if (lane_id() == first) base0 = atomicAdd(&cursor[s0], (uint32_t)__builtin_popcountll(m0));
-  base0 = readlane_u32(base0, first);
+  uint32_t c0 = (uint32_t)__builtin_popcountll(m0);
   uint32_t pos = 0;
-  if (same) pos = base0 + mask_below(m0);
-  else if (active) pos = atomicAdd(&cursor[s], 1u);
+  if (c0 >= kPeelMin) {
+    uint32_t base0 = 0;
+    if (lane_id() == first) base0 = atomicAdd(&cursor[s0], c0);
+    base0 = readlane_u32(base0, first);
+    if (same) pos = base0 + mask_below(m0);
+    else if (active) pos = atomicAdd(&cursor[s], 1u);
+  } else if (active) {
+    pos = atomicAdd(&cursor[s], 1u);
+  }
   return pos;
 }
 

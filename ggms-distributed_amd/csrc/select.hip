@@ -103,6 +103,7 @@ static Config resolve_config(ggms_alg alg, uint64_t n, uint32_t nk, const ggms_o
   c.G_max = std::max<uint32_t>(1, std::min<uint32_t>(bpc * (uint32_t)cus, kMaxWorkgroups));
   c.min_tile = 16384;
   c.max_passes = o.max_passes ? std::min<uint32_t>(o.max_passes, GGMS_MAX_PASS_STATS) : 64;
+  if (c.max_passes < 2) c.max_passes = 2;  // the first pass always hands its k-buckets to a second
   c.fin_cap = sizeof(K) == 4 ? 16384 : 8192;
   c.seed = o.seed;
   c.force_finish = 0;
@@ -203,9 +204,9 @@ struct Prof {
 };
 
 enum Buf {
-  bBufA, bBufB, bRows, bTotals, bRanges0, bRanges1, bCtl, bKs, bKRank, bKState, bKAux, bKList,
+  bBufA, bBufB, bRows, bTotals, bRanges0, bRanges1, bRangesFirst, bCtl, bKs, bKRank, bKState, bKAux, bKList,
   bSlotOfBucket, bSlotDst, bSlotBucket, bActBucket, bActCls, bActDst, bActSlot, bActKlo, bActKhi,
-  bSmallStart, bSmallLen, bSmallKStart, bSmallKFill, bWgFirst, bOut, bCount
+  bSmallStart, bSmallLen, bSmallKStart, bSmallKFill, bWgFirst, bOut, bLut, bSmallKlo, bSmallKhi, bParams1, bCount
 };
 
 template <class T>
@@ -222,13 +223,14 @@ static Layout make_layout(uint64_t n, uint32_t nk, const Config& c) {
   sz[bTotals] = Bm * 4;
   sz[bRanges0] = Rm * sizeof(Range<K>);
   sz[bRanges1] = Rm * sizeof(Range<K>);
+  sz[bRangesFirst] = ((size_t)c.R0_bound + 64) * sizeof(Range<K>);
   sz[bCtl] = (size_t)(c.max_passes + 2) * sizeof(PassCtl);
   sz[bKs] = nkp * 4;
   sz[bKRank] = nkp * 4;
   sz[bKState] = nkp * 4;
   sz[bKAux] = nkp * 4;
   sz[bKList] = nkp * 4;
-  sz[bSlotOfBucket] = Bm * 2;
+  sz[bSlotOfBucket] = Bm * 2 * 2;  // double-buffered by pass parity
   sz[bSlotDst] = Bm * 4;
   sz[bSlotBucket] = Bm * 4;
   sz[bActBucket] = Bm * 4;
@@ -243,6 +245,10 @@ static Layout make_layout(uint64_t n, uint32_t nk, const Config& c) {
   sz[bSmallKFill] = Bm * 4;
   sz[bWgFirst] = (size_t)c.G_max * 4 * 2;  // double-buffered by pass parity
   sz[bOut] = nkp * 8;
+  sz[bLut] = (size_t)lut_cells<K>() * sizeof(LutCell<K>);
+  sz[bParams1] = Bm * sizeof(MapParam<K>);
+  sz[bSmallKlo] = Bm * sizeof(K);
+  sz[bSmallKhi] = Bm * sizeof(K);
   Layout L{};
   size_t o = 0;
   for (int i = 0; i < bCount; ++i) {
@@ -256,12 +262,21 @@ static Layout make_layout(uint64_t n, uint32_t nk, const Config& c) {
 template <class T>
 static size_t count_lds0(const Config& c) {
   using K = typename KeyTraits<T>::K;
-  return ((size_t)c.B0_bound * 4 + 15) / 16 * 16 + (size_t)pow2ceil_host(c.R0_bound) * (sizeof(K) + 16);
+  return ((size_t)c.B0_bound * 4 + 15) / 16 * 16 + Pass0Map<K>::bytes(c.R0_bound);
 }
+
+// LDS ceiling of one workgroup on gfx950 (160 KB per CU; static LDS of the
+// kernels stays under 2 KB).
+constexpr size_t kLdsLimit = 158 * 1024;
 
 static void set_lds_attr(const void* fn, size_t bytes) {
   if (bytes > 65536) (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
+
+#define LDS_CHECK(bytes)                              \
+  do {                                                \
+    if ((bytes) > kLdsLimit) return GGMS_ENOTSUP;     \
+  } while (0)
 
 template <class T>
 static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint64_t n64, const uint32_t* ks,
@@ -290,7 +305,8 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
   K* bufB = (K*)P(bBufB);
   uint32_t* rows = (uint32_t*)P(bRows);
   uint32_t* totals = (uint32_t*)P(bTotals);
-  Range<K>* rng[2] = {(Range<K>*)P(bRanges0), (Range<K>*)P(bRanges1)};
+  Range<K>* rng[2] = {(Range<K>*)P(bRanges0), (Range<K>*)P(bRanges1)};  // passes >= 1, by parity
+  Range<K>* rng_first = (Range<K>*)P(bRangesFirst);                      // pass 0, kept for pass 1
   PassCtl* ctl = (PassCtl*)P(bCtl);
   uint32_t* d_ks = (uint32_t*)P(bKs);
   uint32_t* k_rank = (uint32_t*)P(bKRank);
@@ -342,12 +358,13 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
   if (n <= c.fin_cap) {
     prof.begin(GGMS_KERNEL_OTHER, -1);
     k_tiny_setup<T><<<1, 1024, 0, st>>>(d_vec, n, nk, bufA, k_rank, k_state, (uint32_t*)P(bSmallStart),
-                                        (uint32_t*)P(bSmallLen), (uint32_t*)P(bSmallKStart), klist, ctl);
+                                        (uint32_t*)P(bSmallLen), (K*)P(bSmallKlo), (K*)P(bSmallKhi),
+                                        (uint32_t*)P(bSmallKStart), klist, ctl);
     prof.end();
     HIP_TRY(hipGetLastError());
     prof.begin(GGMS_KERNEL_FINISH, 0);
     k_finish<T><<<1, 1024, fin_lds, st>>>(bufA, ctl, (uint32_t*)P(bSmallStart), (uint32_t*)P(bSmallLen),
-                                          (uint32_t*)P(bSmallKStart), klist, k_rank, k_state, d_out,
+                                          (K*)P(bSmallKlo), (K*)P(bSmallKhi), (uint32_t*)P(bSmallKStart), klist, k_rank, k_state, d_out,
                                           c.fin_cap, &ctl->pad[0]);
     prof.end();
     HIP_TRY(hipGetLastError());
@@ -369,18 +386,22 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
     pa.seed = c.seed;
     pa.mlp = c.mode == kModeLearned ? learned_weights_device(ctx->device) : nullptr;
     size_t lds_p0 = (size_t)pow2ceil_host(c.S) * sizeof(K) + 2048 * sizeof(K);
+    LDS_CHECK(lds_p0);
     set_lds_attr((const void*)k_plan0<T>, lds_p0);
     prof.begin(GGMS_KERNEL_PLAN0, 0);
-    k_plan0<T><<<1, 1024, lds_p0, st>>>(d_vec, pa, rng[0], ctl);
+    k_plan0<T><<<1, 1024, lds_p0, st>>>(d_vec, pa, rng_first, (LutCell<K>*)P(bLut), ctl);
     prof.end();
     HIP_TRY(hipGetLastError());
 
     uint32_t G = G0, B_bound = c.B0_bound, R_bound = c.R0_bound;
+    uint32_t B0 = c.B0_bound, R0 = c.R0_bound;  // exact after pass 0
+    uint16_t* slot_tab[2] = {(uint16_t*)P(bSlotOfBucket),
+                             (uint16_t*)P(bSlotOfBucket) + (std::max<size_t>(c.B0_bound, c.B_cap) + 64)};
     uint32_t U_slot_bound = std::min<uint32_t>(nk, B_bound);
     for (uint32_t pass = 0;; ++pass) {
       PassCtl* cp = ctl + pass;
       PassCtl* cn = ctl + pass + 1;
-      Range<K>* rc = rng[pass & 1];
+      Range<K>* rc = pass == 0 ? rng_first : rng[pass & 1];
       Range<K>* rn = rng[(pass + 1) & 1];
       K* src = (pass & 1) ? bufA : bufB;  // input of pass >= 1
       K* dst = (pass & 1) ? bufB : bufA;  // output of this pass's compaction
@@ -389,15 +410,33 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
       uint32_t* wgf_cur = (uint32_t*)P(bWgFirst) + (size_t)(pass & 1) * c.G_max;
       uint32_t* wgf_next = (uint32_t*)P(bWgFirst) + (size_t)((pass + 1) & 1) * c.G_max;
       const uint32_t NT = c.NT;
+      // pass-1 range parameters live in LDS unless that would push either
+      // pass-1 kernel past 112 KB (then they are read from global memory)
+      uint32_t p1_lds = 0;
+      if (pass == 1) {
+        size_t cnt = ((size_t)B_bound * 4 + 15) / 16 * 16 + pass1_map_bytes<K>(B0, R0, R_bound);
+        size_t cmp = compact_lds_bytes(B_bound, U_slot_bound + 4) + pass1_map_bytes<K>(B0, R0, R_bound);
+        p1_lds = std::max(cnt, cmp) <= 112 * 1024 ? 1u : 0u;
+      }
       // count
       if (pass == 0) {
         size_t lds = count_lds0<T>(c);
+        LDS_CHECK(lds);
         set_lds_attr((const void*)k_count0<T>, lds);
         prof.begin(GGMS_KERNEL_COUNT0, (int)pass);
-        k_count0<T><<<G, NT, lds, st>>>(d_vec, rc, cp, rows);
+        k_count0<T><<<G, NT, lds, st>>>(d_vec, rc, (LutCell<K>*)P(bLut), cp, rows);
+        prof.end();
+      } else if (pass == 1) {
+        size_t lds = ((size_t)B_bound * 4 + 15) / 16 * 16 + pass1_map_bytes<K>(B0, R0, p1_lds ? R_bound : 0);
+        LDS_CHECK(lds);
+        set_lds_attr((const void*)k_count1<K>, lds);
+        prof.begin(GGMS_KERNEL_COUNTS, (int)pass);
+        k_count1<K><<<G, NT, lds, st>>>(src, ctl, (LutCell<K>*)P(bLut), rng_first, slot_tab[0],
+                                        (MapParam<K>*)P(bParams1), p1_lds, cp, rows);
         prof.end();
       } else {
         size_t lds = (size_t)B_bound * 4;
+        LDS_CHECK(lds);
         set_lds_attr((const void*)k_countS<K>, lds);
         prof.begin(GGMS_KERNEL_COUNTS, (int)pass);
         k_countS<K><<<G, NT, lds, st>>>(src, rc, wgf_cur, cp, rows);
@@ -419,7 +458,7 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
       pl.k_rank = k_rank;
       pl.k_state = k_state;
       pl.k_aux = k_aux;
-      pl.slot_of_bucket = (uint16_t*)P(bSlotOfBucket);
+      pl.slot_of_bucket = slot_tab[pass & 1];
       pl.slot_dst = (uint32_t*)P(bSlotDst);
       pl.slot_bucket = (uint32_t*)P(bSlotBucket);
       pl.act_bucket = (uint32_t*)P(bActBucket);
@@ -430,6 +469,8 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
       pl.act_khi = (K*)P(bActKhi);
       pl.small_start = (uint32_t*)P(bSmallStart);
       pl.small_len = (uint32_t*)P(bSmallLen);
+      pl.small_klo = (K*)P(bSmallKlo);
+      pl.small_khi = (K*)P(bSmallKhi);
       pl.small_kstart = (uint32_t*)P(bSmallKStart);
       pl.small_kfill = (uint32_t*)P(bSmallKFill);
       pl.klist = klist;
@@ -441,7 +482,10 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
       pl.G_max = c.G_max;
       pl.min_tile = c.min_tile;
       pl.force_finish = (pass + 1 >= c.max_passes) ? 1u : c.force_finish;
+      pl.merge_small = pass == 0 ? 1u : 0u;
+      pl.next_params = pass == 0 ? (MapParam<K>*)P(bParams1) : nullptr;
       size_t lds_plan = (size_t)B_bound * 8;
+      LDS_CHECK(lds_plan);
       set_lds_attr((const void*)k_plan<T>, lds_plan);
       prof.begin(GGMS_KERNEL_PLAN, (int)pass);
       k_plan<T><<<1, 1024, lds_plan, st>>>(pl);
@@ -449,14 +493,25 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
       HIP_TRY(hipGetLastError());
       // compaction
       if (pass == 0) {
-        size_t lds = ((size_t)B_bound * 2 + 15) / 16 * 16 + (size_t)(U_slot_bound + 4) * 4 +
-                     (size_t)pow2ceil_host(R_bound) * (sizeof(K) + 16);
+        size_t lds = ((size_t)B_bound * 2 + 15) / 16 * 16 + Pass0Map<K>::bytes(R_bound);
+        LDS_CHECK(lds);
         set_lds_attr((const void*)k_compact0<T>, lds);
         prof.begin(GGMS_KERNEL_COMPACT0, (int)pass);
-        k_compact0<T><<<G, NT, lds, st>>>(d_vec, rc, cp, rows, pl.slot_of_bucket, pl.slot_dst, pl.slot_bucket, dst);
+        k_compact0<T><<<G, NT, lds, st>>>(d_vec, rc, (LutCell<K>*)P(bLut), cp, rows, pl.slot_of_bucket, pl.slot_bucket,
+                                          dst);
+        prof.end();
+      } else if (pass == 1) {
+        size_t lds = compact_lds_bytes(B_bound, U_slot_bound + 4) + pass1_map_bytes<K>(B0, R0, p1_lds ? R_bound : 0);
+        LDS_CHECK(lds);
+        set_lds_attr((const void*)k_compact1<K>, lds);
+        prof.begin(GGMS_KERNEL_COMPACTS, (int)pass);
+        k_compact1<K><<<G, NT, lds, st>>>(src, ctl, (LutCell<K>*)P(bLut), rng_first, slot_tab[0],
+                                          (MapParam<K>*)P(bParams1), p1_lds, cp, rows,
+                                          pl.slot_of_bucket, pl.slot_dst, pl.slot_bucket, dst);
         prof.end();
       } else {
-        size_t lds = ((size_t)B_bound * 2 + 15) / 16 * 16 + (size_t)(U_slot_bound + 4) * 4;
+        size_t lds = compact_lds_bytes(B_bound, U_slot_bound + 4);
+        LDS_CHECK(lds);
         set_lds_attr((const void*)k_compactS<K>, lds);
         prof.begin(GGMS_KERNEL_COMPACTS, (int)pass);
         k_compactS<K><<<G, NT, lds, st>>>(src, rc, wgf_cur, cp, rows, pl.slot_of_bucket,
@@ -464,11 +519,14 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
         prof.end();
       }
       HIP_TRY(hipGetLastError());
+      if (pass > 0) {  // the first pass hands every k-bucket to pass 1 (no small segments yet)
       prof.begin(GGMS_KERNEL_FINISH, (int)pass);
-      k_finish<T><<<fin_grid, 1024, fin_lds, st>>>(dst, cp, pl.small_start, pl.small_len, pl.small_kstart, klist,
+      k_finish<T><<<fin_grid, 1024, fin_lds, st>>>(dst, cp, pl.small_start, pl.small_len, pl.small_klo,
+                                                   pl.small_khi, pl.small_kstart, klist,
                                                    k_rank, k_state, d_out, c.fin_cap, &cp->pad[0]);
       prof.end();
       HIP_TRY(hipGetLastError());
+      }
       HIP_TRY(hipMemcpyAsync(&hctl[0], cp, 2 * ctlb, hipMemcpyDeviceToHost, st));
       HIP_TRY(hipStreamSynchronize(st));
       const PassCtl& h = hctl[0];
@@ -476,6 +534,8 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
       if (pass == 0) {
         if (h.bad_k) return GGMS_EINVAL;
         if (h.nan_count) return GGMS_ENAN;
+        B0 = h.B;
+        R0 = h.R;
       }
       if (pass < GGMS_MAX_PASS_STATS) {
         stt.n_in[pass] = h.n;
@@ -553,12 +613,14 @@ static ggms_status run_first_pass(ggms_ctx* ctx, const T* d_vec, uint64_t n64, c
   pa.seed = c.seed;
   pa.mlp = c.mode == kModeLearned ? learned_weights_device(ctx->device) : nullptr;
   size_t lds_p0 = (size_t)pow2ceil_host(c.S) * sizeof(K) + 2048 * sizeof(K);
+  LDS_CHECK(lds_p0);
   set_lds_attr((const void*)k_plan0<T>, lds_p0);
-  k_plan0<T><<<1, 1024, lds_p0, st>>>(d_vec, pa, rng, ctl);
+  k_plan0<T><<<1, 1024, lds_p0, st>>>(d_vec, pa, rng, (LutCell<K>*)P(bLut), ctl);
   HIP_TRY(hipGetLastError());
   size_t lds = count_lds0<T>(c);
+  LDS_CHECK(lds);
   set_lds_attr((const void*)k_count0<T>, lds);
-  k_count0<T><<<G0, c.NT, lds, st>>>(d_vec, rng, ctl, rows);
+  k_count0<T><<<G0, c.NT, lds, st>>>(d_vec, rng, (LutCell<K>*)P(bLut), ctl, rows);
   HIP_TRY(hipGetLastError());
   k_rowscan<<<(c.B0_bound + 63) / 64, 1024, 0, st>>>(rows, ctl, totals);
   HIP_TRY(hipGetLastError());

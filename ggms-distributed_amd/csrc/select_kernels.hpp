@@ -226,49 +226,132 @@ __device__ __forceinline__ void stream_tile_any(const V* __restrict__ x, uint32_
 }
 
 // ---------------------------------------------------------------------------
-// pass-0 range table in LDS: sorted range starts + map parameters
+// First-pass map in LDS.  A 4096-entry table indexed by the top 12 key bits
+// describes every key-space cell: its first range r0 and, when exactly one
+// range starts inside the cell, that start's offset from the cell's first key.
+// The range of a key is then  r0 + (low bits of key >= offset)  with no
+// branch; only cells holding two or more range starts (flagged) fall back to
+// a short binary search, guarded by a wave-uniform test.  The map parameters
+// of a range are packed for one 16-byte (u32 keys) or 32-byte (u64 keys) LDS
+// read.
 // ---------------------------------------------------------------------------
-template <class K> struct LdsRanges {
-  K* lo;
-  uint32_t* base;
-  uint32_t* sh;
-  uint32_t* mult;
-  uint32_t* nbm1;
-  uint32_t R, Rp;
+// 4096 cells of 8 bytes for u32 keys, 2048 cells of 16 bytes for u64 keys
+// (32 KB of LDS either way).
+template <class K> __host__ __device__ __forceinline__ constexpr uint32_t lut_bits() { return sizeof(K) == 4 ? 12u : 11u; }
+template <class K> __host__ __device__ __forceinline__ constexpr uint32_t lut_cells() { return 1u << lut_bits<K>(); }
+constexpr uint32_t kLutMulti = 0x80000000u;
 
-  __device__ static size_t bytes(uint32_t Rp) { return (size_t)Rp * (sizeof(K) + 16); }
-  __device__ void carve(unsigned char* p, uint32_t Rp_) {
-    Rp = Rp_;
-    lo = reinterpret_cast<K*>(p);
-    base = reinterpret_cast<uint32_t*>(p + (size_t)Rp * sizeof(K));
-    sh = base + Rp;
-    mult = sh + Rp;
-    nbm1 = mult + Rp;
+template <class K> struct MapParam;
+template <> struct __align__(16) MapParam<uint32_t> {
+  uint32_t lo, base, mult, sh;
+};
+template <> struct __align__(16) MapParam<uint64_t> {
+  uint64_t lo;
+  uint32_t base, mult, sh, pad0, pad1, pad2;
+};
+
+template <class K> struct LutCell;
+template <> struct __align__(8) LutCell<uint32_t> {
+  uint32_t boff;  // offset of the single range start inside the cell, or the cell size
+  uint32_t r;     // r0 | r1 << 15 | kLutMulti when >= 2 starts
+};
+template <> struct __align__(16) LutCell<uint64_t> {
+  uint64_t boff;
+  uint32_t r, pad;
+};
+
+template <class K> __host__ __device__ __forceinline__ constexpr uint32_t lut_shift() { return sizeof(K) * 8 - lut_bits<K>(); }
+template <class K> __device__ __forceinline__ uint32_t lut_cell(K key) { return (uint32_t)(key >> lut_shift<K>()); }
+
+template <class K> __device__ __forceinline__ MapParam<K> to_param(const Range<K>& r) {
+  MapParam<K> q{};
+  q.lo = r.lo;
+  q.base = r.base;
+  q.mult = r.mult;
+  q.sh = r.sh;
+  return q;
+}
+
+template <class K> __device__ __forceinline__ uint32_t param_bucket(const MapParam<K>& q, K key) {
+  K d = key - q.lo;
+  uint32_t t = sizeof(K) == 4 ? (uint32_t)d : (uint32_t)(d >> q.sh);
+  uint32_t off = q.mult ? __umulhi(t, q.mult) : t;
+  return q.base + off;
+}
+
+template <class K> struct Pass0Map {
+  LutCell<K>* lut;
+  MapParam<K>* prm;
+
+  static __host__ __device__ size_t bytes(uint32_t R) {
+    return (size_t)lut_cells<K>() * sizeof(LutCell<K>) + (size_t)R * sizeof(MapParam<K>);
   }
-  __device__ void load(const Range<K>* __restrict__ g, uint32_t R_) {
-    R = R_;
-    for (uint32_t i = threadIdx.x; i < Rp; i += blockDim.x) {
-      if (i < R) {
-        Range<K> r = g[i];
-        lo[i] = r.lo; base[i] = r.base; sh[i] = r.sh; mult[i] = r.mult; nbm1[i] = r.nb - 1;
-      } else {
-        lo[i] = KeyMax<K>::v;
-        base[i] = 0; sh[i] = 0; mult[i] = 0; nbm1[i] = 0;
+  __device__ void carve(unsigned char* p) {
+    lut = reinterpret_cast<LutCell<K>*>(p);
+    prm = reinterpret_cast<MapParam<K>*>(p + (size_t)lut_cells<K>() * sizeof(LutCell<K>));
+  }
+  __device__ void load(const LutCell<K>* __restrict__ g_lut, const Range<K>* __restrict__ g, uint32_t R) {
+    for (uint32_t i = threadIdx.x; i < lut_cells<K>(); i += blockDim.x) lut[i] = g_lut[i];
+    for (uint32_t i = threadIdx.x; i < R; i += blockDim.x) prm[i] = to_param<K>(g[i]);
+  }
+  __device__ __forceinline__ uint32_t range_of(K key) const {
+    const LutCell<K> e = lut[lut_cell<K>(key)];
+    const K low = key & (K)(((K)1 << lut_shift<K>()) - 1);
+    uint32_t r = (e.r & 0x7FFFu) + (low >= e.boff ? 1u : 0u);
+    bool multi = (e.r & kLutMulti) != 0;
+    if (wave_ballot(multi)) {
+      if (multi) {
+        uint32_t lo = e.r & 0x7FFFu, hi = (e.r >> 15) & 0x7FFFu;
+        while (lo < hi) {
+          uint32_t mid = (lo + hi + 1) >> 1;
+          if (prm[mid].lo <= key) lo = mid; else hi = mid - 1;
+        }
+        r = lo;
       }
     }
+    return r;
   }
-  __device__ __forceinline__ uint32_t bucket(K key) const {
-    uint32_t r = 0;
-    for (uint32_t step = Rp >> 1; step; step >>= 1) {
-      uint32_t c = r + step;
-      if (lo[c] <= key) r = c;
-    }
-    r = r < R ? r : R - 1;
-    uint32_t off = map_off<K>(key, lo[r], sh[r], mult[r]);
-    off = off < nbm1[r] ? off : nbm1[r];
-    return base[r] + off;
-  }
+  __device__ __forceinline__ uint32_t bucket(K key) const { return param_bucket<K>(prm[range_of(key)], key); }
 };
+
+// Cell table of a range list (ranges partition the key space, sorted by lo).
+template <class K>
+__device__ void build_lut(const Range<K>* __restrict__ ranges, uint32_t R, LutCell<K>* __restrict__ g_lut) {
+  constexpr uint32_t sh = lut_shift<K>();
+  for (uint32_t c = threadIdx.x; c < lut_cells<K>(); c += blockDim.x) {
+    K c0 = (K)((K)c << sh);
+    K c1 = (K)(c0 + (((K)1 << sh) - 1));
+    uint32_t r0, r1;
+    {
+      uint32_t lo = 0, hi = R - 1;
+      while (lo < hi) {
+        uint32_t mid = (lo + hi + 1) >> 1;
+        if (ranges[mid].lo <= c0) lo = mid; else hi = mid - 1;
+      }
+      r0 = lo;
+    }
+    {
+      uint32_t lo = r0, hi = R - 1;
+      while (lo < hi) {
+        uint32_t mid = (lo + hi + 1) >> 1;
+        if (ranges[mid].lo <= c1) lo = mid; else hi = mid - 1;
+      }
+      r1 = lo;
+    }
+    LutCell<K> e{};
+    if (r1 == r0) {
+      e.boff = (K)((K)1 << sh);  // no start inside: the compare never fires
+      e.r = r0;
+    } else if (r1 == r0 + 1) {
+      e.boff = (K)(ranges[r1].lo - c0);
+      e.r = r0;
+    } else {
+      e.boff = (K)((K)1 << sh);
+      e.r = r0 | (r1 << 15) | kLutMulti;
+    }
+    g_lut[c] = e;
+  }
+}
 
 // Plain fill (used instead of hipMemsetAsync so the sequence stays capturable).
 __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t words, uint32_t v) {
@@ -299,23 +382,21 @@ __global__ void k_init_ks(const uint32_t* __restrict__ ks, uint32_t nk, uint32_t
 // ---------------------------------------------------------------------------
 template <class T>
 __global__ void __launch_bounds__(1024) k_count0(const T* __restrict__ x, const Range<typename KeyTraits<T>::K>* __restrict__ ranges,
+                                                 const LutCell<typename KeyTraits<T>::K>* __restrict__ lut,
                                                  const PassCtl* __restrict__ ctl, uint32_t* __restrict__ rows) {
   using K = typename KeyTraits<T>::K;
   extern __shared__ __align__(16) unsigned char smem[];
   const uint32_t B = ctl->B, n = ctl->n, tile = ctl->tile;
   if (blockIdx.x >= ctl->G) return;
   uint32_t* hist = reinterpret_cast<uint32_t*>(smem);
-  LdsRanges<K> rt;
-  rt.carve(smem + ((size_t)B * 4 + 15) / 16 * 16, ctl->Rp);
+  Pass0Map<K> mp;
+  mp.carve(smem + ((size_t)B * 4 + 15) / 16 * 16);
   for (uint32_t i = threadIdx.x; i < B; i += blockDim.x) hist[i] = 0;
-  rt.load(ranges, ctl->R);
+  mp.load(lut, ranges, ctl->R);
   __syncthreads();
   uint32_t t0 = blockIdx.x * tile;
   uint32_t t1 = t0 + tile < n ? t0 + tile : n;
-  stream_tile_any<T>(x, t0, t1, [&](T v, uint32_t) {
-    uint32_t b = rt.bucket(KeyTraits<T>::enc(v));
-    lds_count_peeled(hist, b, true);
-  });
+  stream_tile_any<T>(x, t0, t1, [&](T v, uint32_t) { lds_count(hist, mp.bucket(KeyTraits<T>::enc(v))); });
   __syncthreads();
   uint32_t* row = rows + (size_t)blockIdx.x * B;
   for (uint32_t i = threadIdx.x; i < B; i += blockDim.x) row[i] = hist[i];
@@ -340,9 +421,7 @@ template <class K> struct SegCursor {
   }
   __device__ __forceinline__ uint32_t bucket(K key, uint32_t pos) {
     while (pos >= seg_end) load(r + 1);
-    uint32_t off = map_off<K>(key, lo, sh, mult);
-    off = off < nbm1 ? off : nbm1;
-    return base + off;
+    return base + map_off<K>(key, lo, sh, mult);
   }
 };
 
@@ -362,8 +441,7 @@ __global__ void __launch_bounds__(1024) k_countS(const K* __restrict__ buf, cons
   cur.rg = ranges;
   cur.load(wg_first[blockIdx.x]);
   stream_tile<K>(buf, t0, t1, [&](K key, uint32_t pos) {
-    uint32_t b = cur.bucket(key, pos);
-    lds_count_peeled(hist, b, true);
+    lds_count(hist, cur.bucket(key, pos));
   });
   __syncthreads();
   uint32_t* row = rows + (size_t)blockIdx.x * B;
@@ -424,6 +502,8 @@ template <class K> struct PlanArgs {
   K* act_khi;
   uint32_t* small_start;     // [U_small]
   uint32_t* small_len;
+  K* small_klo;              // key interval of each small segment
+  K* small_khi;
   uint32_t* small_kstart;    // [U_small + 1]
   uint32_t* small_kfill;     // [U_small]
   uint32_t* klist;           // [nk]
@@ -435,6 +515,8 @@ template <class K> struct PlanArgs {
   uint32_t G_max;            // workgroups of the streaming kernels
   uint32_t min_tile;         // smallest tile worth a workgroup
   uint32_t force_finish;     // nonzero: stop after this pass (all active buckets must fit fin_cap)
+  uint32_t merge_small;      // nonzero: small buckets also become next-pass ranges (one bucket each)
+  MapParam<K>* next_params;  // packed map parameters of next_ranges (or null)
 };
 
 template <class T>
@@ -504,6 +586,7 @@ __global__ void __launch_bounds__(1024) k_plan(PlanArgs<typename KeyTraits<T>::K
     bucket_interval<K>(rg, b - rg.base, klo, khi);
     uint32_t cls;
     if (klo == khi) cls = kResolved;
+    else if (a.merge_small) cls = kClsBig;
     else if (c <= a.fin_cap) cls = kClsSmall;
     else cls = a.force_finish ? kClsSmall : kClsBig;
     a.act_bucket[q] = b;
@@ -573,7 +656,8 @@ __global__ void __launch_bounds__(1024) k_plan(PlanArgs<typename KeyTraits<T>::K
       nr.seg_end = dst + c;
       nr.base = 0;
       uint32_t nb;
-      if (2 * U_big <= Bn) nb = 2 + (uint32_t)(((uint64_t)c * (Bn - 2 * U_big)) / N_big);
+      if (c <= a.fin_cap) nb = 1;  // merged small bucket: carried whole, finished after the next pass
+      else if (2 * U_big <= Bn) nb = 2 + (uint32_t)(((uint64_t)c * (Bn - 2 * U_big)) / N_big);
       else nb = 1 + (uint32_t)(((uint64_t)c * (Bn - U_big)) / N_big);
       make_map<K>(nr, nb);
       a.next_ranges[s] = nr;
@@ -581,6 +665,8 @@ __global__ void __launch_bounds__(1024) k_plan(PlanArgs<typename KeyTraits<T>::K
       uint32_t j = s - U_big;
       a.small_start[j] = dst;
       a.small_len[j] = c;
+      a.small_klo[j] = a.act_klo[q];
+      a.small_khi[j] = a.act_khi[q];
       a.small_kfill[j] = 0;
     }
   }
@@ -598,6 +684,11 @@ __global__ void __launch_bounds__(1024) k_plan(PlanArgs<typename KeyTraits<T>::K
     uint32_t off = block_excl_scan_u32(s, scratch, &tot);
     for (uint32_t j = j0; j < j1; ++j) {
       a.next_ranges[j].base = off;
+      if (a.next_params) {
+        Range<K> r = a.next_ranges[j];
+        r.base = off;
+        a.next_params[j] = to_param<K>(r);
+      }
       off += a.next_ranges[j].nb;
     }
     B_next = tot;
@@ -723,35 +814,146 @@ __device__ __forceinline__ void compact_init(CompactTables& ct, unsigned char* p
     ct.cursor[s] = slot_dst[s] + row_prefix[slot_bucket[s]];
 }
 
-__device__ __forceinline__ size_t compact_lds_bytes(uint32_t B, uint32_t U_slots) {
-  return ((size_t)B * 2 + 15) / 16 * 16 + (size_t)U_slots * 4;
+__host__ __device__ __forceinline__ size_t compact_lds_bytes(uint32_t B, uint32_t U_slots) {
+  return ((size_t)B * 2 + 15) / 16 * 16 + ((size_t)U_slots * 4 + 15) / 16 * 16;
 }
 
+// First-pass compaction, append layout: workgroup w writes the survivors of
+// its tile (elements of any active bucket) contiguously to
+//   [R_w, R_w + n_w),  R_w = sum over active buckets b of rows_prefix[w][b]
+// (survivors of all earlier workgroups; the sum telescopes, so no atomics).
+// Each wave claims a run for its survivors with one LDS atomic and writes it
+// with consecutive lanes, so HBM sees full lines instead of one partial line
+// per survivor.  The next pass finds each element's range from its key.
 template <class T>
 __global__ void __launch_bounds__(1024) k_compact0(const T* __restrict__ x, const Range<typename KeyTraits<T>::K>* __restrict__ ranges,
+                                                   const LutCell<typename KeyTraits<T>::K>* __restrict__ lut,
                                                    const PassCtl* __restrict__ ctl, const uint32_t* __restrict__ rows,
                                                    const uint16_t* __restrict__ slot_of_bucket,
-                                                   const uint32_t* __restrict__ slot_dst,
                                                    const uint32_t* __restrict__ slot_bucket,
                                                    typename KeyTraits<T>::K* __restrict__ out) {
   using K = typename KeyTraits<T>::K;
   extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ uint32_t cursor;
+  __shared__ uint32_t part[16];
   const uint32_t B = ctl->B, n = ctl->n, tile = ctl->tile, U_slots = ctl->U_slots;
   if (blockIdx.x >= ctl->G || U_slots == 0) return;
-  CompactTables ct;
-  compact_init(ct, smem, B, U_slots, slot_of_bucket, slot_dst, slot_bucket, rows + (size_t)blockIdx.x * B);
-  LdsRanges<K> rt;
-  rt.carve(smem + compact_lds_bytes(B, U_slots + 4), ctl->Rp);
-  rt.load(ranges, ctl->R);
+  uint16_t* slot = reinterpret_cast<uint16_t*>(smem);
+  Pass0Map<K> mp;
+  mp.carve(smem + ((size_t)B * 2 + 15) / 16 * 16);
+  for (uint32_t i = threadIdx.x; i < B; i += blockDim.x) slot[i] = slot_of_bucket[i];
+  mp.load(lut, ranges, ctl->R);
+  const uint32_t* rp = rows + (size_t)blockIdx.x * B;
+  uint32_t acc = 0;
+  for (uint32_t s = threadIdx.x; s < U_slots; s += blockDim.x) acc += rp[slot_bucket[s]];
+  {
+    uint32_t tot;
+    (void)block_excl_scan_u32(acc, part, &tot);
+    if (threadIdx.x == 0) cursor = tot;
+  }
   __syncthreads();
   uint32_t t0 = blockIdx.x * tile;
   uint32_t t1 = t0 + tile < n ? t0 + tile : n;
   stream_tile_any<T>(x, t0, t1, [&](T v, uint32_t) {
     K key = KeyTraits<T>::enc(v);
-    uint32_t s = ct.slot[rt.bucket(key)];
+    bool act = slot[mp.bucket(key)] != kNone16;
+    uint64_t m = wave_ballot(act);
+    if (m) {
+      uint32_t first = (uint32_t)__builtin_ctzll(m);
+      uint32_t base = 0;
+      if (lane_id() == first) base = atomicAdd(&cursor, (uint32_t)__builtin_popcountll(m));
+      base = readlane_u32(base, first);
+      if (act) out[base + mask_below(m)] = key;
+    }
+  });
+}
+
+// Pass 1 reads the append-layout buffer: an element's range is the slot of its
+// first-pass bucket (recomputed from the key with the first-pass map), and the
+// range's own map gives the pass-1 bucket.  The pass-1 ranges' parameters are
+// staged in LDS, indexed by slot.
+template <class K> struct Pass1Map {
+  Pass0Map<K> m0;
+  const uint16_t* slot0;               // LDS: first-pass bucket -> slot (= pass-1 range)
+  const MapParam<K>* p1;               // pass-1 range parameters by slot (LDS when they fit)
+  __device__ __forceinline__ uint32_t bucket(K key) const {
+    uint32_t s0 = slot0[m0.bucket(key)];
+    return param_bucket<K>(p1[s0], key);
+  }
+};
+
+// LDS bytes of the pass-1 map; R1 = 0 when the parameters stay in global memory.
+template <class K>
+__host__ __device__ __forceinline__ size_t pass1_map_bytes(uint32_t B0, uint32_t R0, uint32_t R1) {
+  return ((size_t)B0 * 2 + 15) / 16 * 16 + Pass0Map<K>::bytes(R0) + (size_t)R1 * sizeof(MapParam<K>);
+}
+
+template <class K>
+__device__ __forceinline__ void pass1_map_init(Pass1Map<K>& pm, unsigned char* p, const PassCtl* ctl0,
+                                               const LutCell<K>* lut, const Range<K>* rng0, const uint16_t* slot0_g,
+                                               const MapParam<K>* p1_g, uint32_t R1, bool p1_in_lds) {
+  const uint32_t B0 = ctl0->B, R0 = ctl0->R;
+  uint16_t* s0 = reinterpret_cast<uint16_t*>(p);
+  for (uint32_t i = threadIdx.x; i < B0; i += blockDim.x) s0[i] = slot0_g[i];
+  pm.slot0 = s0;
+  unsigned char* q = p + ((size_t)B0 * 2 + 15) / 16 * 16;
+  pm.m0.carve(q);
+  pm.m0.load(lut, rng0, R0);
+  if (p1_in_lds) {
+    MapParam<K>* p1 = reinterpret_cast<MapParam<K>*>(q + Pass0Map<K>::bytes(R0));
+    for (uint32_t i = threadIdx.x; i < R1; i += blockDim.x) p1[i] = p1_g[i];
+    pm.p1 = p1;
+  } else {
+    pm.p1 = p1_g;
+  }
+}
+
+template <class K>
+__global__ void __launch_bounds__(1024) k_count1(const K* __restrict__ buf, const PassCtl* __restrict__ ctl0,
+                                                 const LutCell<K>* __restrict__ lut, const Range<K>* __restrict__ rng0,
+                                                 const uint16_t* __restrict__ slot0, const MapParam<K>* __restrict__ p1,
+                                                 uint32_t p1_in_lds, const PassCtl* __restrict__ ctl,
+                                                 uint32_t* __restrict__ rows) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const uint32_t B = ctl->B, n = ctl->n, tile = ctl->tile;
+  if (blockIdx.x >= ctl->G) return;
+  uint32_t* hist = reinterpret_cast<uint32_t*>(smem);
+  Pass1Map<K> pm;
+  pass1_map_init<K>(pm, smem + ((size_t)B * 4 + 15) / 16 * 16, ctl0, lut, rng0, slot0, p1, ctl->R, p1_in_lds != 0);
+  for (uint32_t i = threadIdx.x; i < B; i += blockDim.x) hist[i] = 0;
+  __syncthreads();
+  uint32_t t0 = blockIdx.x * tile;
+  uint32_t t1 = t0 + tile < n ? t0 + tile : n;
+  stream_tile<K>(buf, t0, t1, [&](K key, uint32_t) { lds_count(hist, pm.bucket(key)); });
+  __syncthreads();
+  uint32_t* row = rows + (size_t)blockIdx.x * B;
+  for (uint32_t i = threadIdx.x; i < B; i += blockDim.x) row[i] = hist[i];
+}
+
+template <class K>
+__global__ void __launch_bounds__(1024) k_compact1(const K* __restrict__ buf, const PassCtl* __restrict__ ctl0,
+                                                   const LutCell<K>* __restrict__ lut, const Range<K>* __restrict__ rng0,
+                                                   const uint16_t* __restrict__ slot0, const MapParam<K>* __restrict__ p1,
+                                                   uint32_t p1_in_lds, const PassCtl* __restrict__ ctl,
+                                                   const uint32_t* __restrict__ rows,
+                                                   const uint16_t* __restrict__ slot_of_bucket,
+                                                   const uint32_t* __restrict__ slot_dst,
+                                                   const uint32_t* __restrict__ slot_bucket, K* __restrict__ out) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const uint32_t B = ctl->B, n = ctl->n, tile = ctl->tile, U_slots = ctl->U_slots;
+  if (blockIdx.x >= ctl->G || U_slots == 0) return;
+  CompactTables ct;
+  compact_init(ct, smem, B, U_slots, slot_of_bucket, slot_dst, slot_bucket, rows + (size_t)blockIdx.x * B);
+  Pass1Map<K> pm;
+  pass1_map_init<K>(pm, smem + compact_lds_bytes(B, U_slots + 4), ctl0, lut, rng0, slot0, p1, ctl->R, p1_in_lds != 0);
+  __syncthreads();
+  uint32_t t0 = blockIdx.x * tile;
+  uint32_t t1 = t0 + tile < n ? t0 + tile : n;
+  stream_tile<K>(buf, t0, t1, [&](K key, uint32_t) {
+    uint32_t s = ct.slot[pm.bucket(key)];
     bool act = s != kNone16;
-    uint32_t pos = lds_claim_peeled(ct.cursor, s, act);
-    if (act) out[pos] = key;
+    uint32_t p = lds_claim(ct.cursor, s, act);
+    if (act) out[p] = key;
   });
 }
 
@@ -776,24 +978,76 @@ __global__ void __launch_bounds__(1024) k_compactS(const K* __restrict__ buf, co
   stream_tile<K>(buf, t0, t1, [&](K key, uint32_t pos) {
     uint32_t s = ct.slot[cur.bucket(key, pos)];
     bool act = s != kNone16;
-    uint32_t p = lds_claim_peeled(ct.cursor, s, act);
+    uint32_t p = lds_claim(ct.cursor, s, act);
     if (act) out[p] = key;
   });
 }
 
 // ---------------------------------------------------------------------------
 // k_finish: exact answers for the ks of every small segment.
+// The segment (<= fin_cap keys) is loaded into LDS once.  A segment holding
+// few ks answers each by radix select: 8-bit digits over only the low bits in
+// which the segment's key interval [klo, khi] varies (all keys share the
+// bits above), one LDS histogram + one wave scan per digit.  A segment holding
+// many ks is sorted instead (bitonic network in LDS) and read at every rank.
 // ---------------------------------------------------------------------------
+constexpr uint32_t kSelectMaxKs = 8;
+
+template <class K>
+__device__ K radix_select_lds(const K* __restrict__ s, uint32_t len, K klo, K khi, uint32_t rank, uint32_t* hist,
+                              uint32_t* bcast) {
+  constexpr uint32_t kb = sizeof(K) * 8;
+  uint32_t vary = bitlen64((uint64_t)(klo ^ khi));
+  K prefix = vary >= kb ? (K)0 : (K)(klo & (K)(~(K)0 << vary));
+  uint32_t top = (vary + 7) & ~7u;
+  for (int shift = (int)top - 8; shift >= 0; shift -= 8) {
+    for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) hist[i] = 0;
+    __syncthreads();
+    K hmask = (uint32_t)shift + 8 >= kb ? (K)0 : (K)(~(K)0 << (shift + 8));
+    K want = prefix & hmask;
+    for (uint32_t i = threadIdx.x; i < len; i += blockDim.x) {
+      K key = s[i];
+      bool m = (key & hmask) == want;
+      lds_count(hist, (uint32_t)(key >> shift) & 255u, m);
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) {
+      uint32_t l = threadIdx.x;
+      uint32_t c0 = hist[4 * l], c1 = hist[4 * l + 1], c2 = hist[4 * l + 2], c3 = hist[4 * l + 3];
+      uint32_t sum = c0 + c1 + c2 + c3;
+      uint32_t cum = wave_incl_scan_u32(sum) - sum;
+      uint32_t cs[4] = {c0, c1, c2, c3};
+      for (uint32_t b = 0; b < 4; ++b) {
+        if (cum < rank && cum + cs[b] >= rank) {
+          bcast[0] = 4 * l + b;
+          bcast[1] = cum;
+        }
+        cum += cs[b];
+      }
+    }
+    __syncthreads();
+    uint32_t d = bcast[0];
+    rank -= bcast[1];
+    prefix = (K)(prefix | ((K)d << shift));
+    __syncthreads();
+  }
+  return prefix;
+}
+
 template <class T>
 __global__ void __launch_bounds__(1024) k_finish(const typename KeyTraits<T>::K* __restrict__ buf, const PassCtl* __restrict__ ctl,
                                                  const uint32_t* __restrict__ small_start,
                                                  const uint32_t* __restrict__ small_len,
+                                                 const typename KeyTraits<T>::K* __restrict__ small_klo,
+                                                 const typename KeyTraits<T>::K* __restrict__ small_khi,
                                                  const uint32_t* __restrict__ small_kstart,
                                                  const uint32_t* __restrict__ klist, uint32_t* __restrict__ k_rank,
                                                  uint32_t* __restrict__ k_state, T* __restrict__ out,
                                                  uint32_t fin_cap, uint32_t* __restrict__ err) {
   using K = typename KeyTraits<T>::K;
   extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ uint32_t hist[256];
+  __shared__ uint32_t bcast[2];
   K* s = reinterpret_cast<K*>(smem);
   const uint32_t U_small = ctl->U_small;
   for (uint32_t j = blockIdx.x; j < U_small; j += gridDim.x) {
@@ -802,15 +1056,29 @@ __global__ void __launch_bounds__(1024) k_finish(const typename KeyTraits<T>::K*
       if (threadIdx.x == 0) atomicAdd(err, 1u);
       continue;
     }
-    uint32_t M = pow2ceil_u32(len);
-    for (uint32_t i = threadIdx.x; i < M; i += blockDim.x) s[i] = i < len ? buf[st + i] : KeyMax<K>::v;
-    __syncthreads();
-    bitonic_sort_lds<K>(s, M);
-    for (uint32_t q = small_kstart[j] + threadIdx.x; q < small_kstart[j + 1]; q += blockDim.x) {
-      uint32_t k = klist[q];
-      uint32_t r = k_rank[k] - 1;
-      out[k] = KeyTraits<T>::dec(s[r]);
-      k_state[k] = kDone;
+    uint32_t q0 = small_kstart[j], q1 = small_kstart[j + 1];
+    if (q1 - q0 <= kSelectMaxKs) {
+      for (uint32_t i = threadIdx.x; i < len; i += blockDim.x) s[i] = buf[st + i];
+      __syncthreads();
+      K klo = small_klo[j], khi = small_khi[j];
+      for (uint32_t q = q0; q < q1; ++q) {
+        uint32_t k = klist[q];
+        K key = radix_select_lds<K>(s, len, klo, khi, k_rank[k], hist, bcast);
+        if (threadIdx.x == 0) {
+          out[k] = KeyTraits<T>::dec(key);
+          k_state[k] = kDone;
+        }
+      }
+    } else {
+      uint32_t M = pow2ceil_u32(len);
+      for (uint32_t i = threadIdx.x; i < M; i += blockDim.x) s[i] = i < len ? buf[st + i] : KeyMax<K>::v;
+      __syncthreads();
+      bitonic_sort_lds<K>(s, M);
+      for (uint32_t q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
+        uint32_t k = klist[q];
+        out[k] = KeyTraits<T>::dec(s[k_rank[k] - 1]);
+        k_state[k] = kDone;
+      }
     }
     __syncthreads();
   }
@@ -823,6 +1091,7 @@ template <class T>
 __global__ void k_tiny_setup(const T* __restrict__ x, uint32_t n, uint32_t nk, typename KeyTraits<T>::K* __restrict__ buf,
                              uint32_t* __restrict__ k_rank, uint32_t* __restrict__ k_state,
                              uint32_t* __restrict__ small_start, uint32_t* __restrict__ small_len,
+                             typename KeyTraits<T>::K* __restrict__ small_klo, typename KeyTraits<T>::K* __restrict__ small_khi,
                              uint32_t* __restrict__ small_kstart, uint32_t* __restrict__ klist, PassCtl* ctl) {
   for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) buf[i] = KeyTraits<T>::enc(x[i]);
   __shared__ uint32_t scratch[32];
@@ -838,6 +1107,8 @@ __global__ void k_tiny_setup(const T* __restrict__ x, uint32_t n, uint32_t nk, t
   if (threadIdx.x == 0) {
     small_start[0] = 0;
     small_len[0] = n;
+    small_klo[0] = 0;
+    small_khi[0] = KeyMax<typename KeyTraits<T>::K>::v;
     small_kstart[0] = 0;
     small_kstart[1] = tot;
     ctl->U_small = 1;

@@ -80,10 +80,12 @@ __device__ K learned_pivot(const K* s, uint32_t S, double rank_frac, const float
 template <class T>
 __global__ void __launch_bounds__(1024) k_plan0(const T* __restrict__ x, Plan0Args a,
                                                 Range<typename KeyTraits<T>::K>* __restrict__ ranges,
+                                                LutCell<typename KeyTraits<T>::K>* __restrict__ lut,
                                                 PassCtl* __restrict__ ctl) {
   using K = typename KeyTraits<T>::K;
   using KT = KeyTraits<T>;
   extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ uint32_t bcast_R;
   const uint32_t n = ctl->n;
   const uint32_t S = a.S;
   const uint32_t Spad = pow2ceil_u32(S);
@@ -219,7 +221,12 @@ __global__ void __launch_bounds__(1024) k_plan0(const T* __restrict__ x, Plan0Ar
     ctl->B = base;
     ctl->nan_lo_bucket = nan_lo;
     ctl->nan_hi_bucket = nan_hi;
+    bcast_R = R;
   }
+  __threadfence_block();
+  __syncthreads();
+  // 4. key-prefix cell table of the ranges
+  build_lut<K>(ranges, bcast_R, lut);
 }
 
 }  // namespace ggms
