@@ -119,17 +119,22 @@ def main():
         "step_ms_median": round(1000 * statistics.median(t_lin), 4),
     }
 
-    # learned-splitter form (same engine, MLP pivots) when weights are present
-    try:
-        l_opts = ggms.default_opts(splitter="learned")
-        out_l, t_l, tall_l = timed(l_opts, args.steps, args.warmup)
-        result["learned"] = {"value": round(n / (tall_l / args.steps), 1),
-                             "ms_per_step": round(1000.0 * tall_l / args.steps, 4),
-                             "speedup_vs_linear": round(tall_lin / tall_l, 4),
-                             "identical_outputs": bool(np.array_equal(out_l.view(np.uint32),
-                                                                      out_lin.view(np.uint32)))}
-    except ggms.GGMSError as e:
-        result["learned"] = {"value": None, "note": str(e)}
+    # learned-splitter form (same engine, MLP pivots) and its no-network ablation
+    for form in ("learned", "sketch"):
+        try:
+            f_opts = ggms.default_opts(splitter=form)
+            out_f, t_f, tall_f = timed(f_opts, args.steps, args.warmup)
+            ctx.select(x, ks, "iterative", ggms.default_opts(splitter=form, profile=1))
+            sf = ctx.stats()
+            result[form] = {"value": round(n / (tall_f / args.steps), 1),
+                            "ms_per_step": round(1000.0 * tall_f / args.steps, 4),
+                            "speedup_vs_linear": round(tall_lin / tall_f, 4),
+                            "identical_outputs": bool(np.array_equal(out_f.view(np.uint32),
+                                                                     out_lin.view(np.uint32))),
+                            "n_kept": sf["n_kept"], "buckets": sf["buckets"],
+                            "kernel_ms": {k: round(v, 4) for k, v in sf["kernel_ms"].items() if v > 0}}
+        except ggms.GGMSError as e:
+            result[form] = {"value": None, "note": str(e)}
 
     if not args.no_check:
         import oracle

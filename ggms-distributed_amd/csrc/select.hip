@@ -74,13 +74,16 @@ static uint32_t pow2ceil_host(uint32_t v) {
 template <class K>
 static Config resolve_config(ggms_alg alg, uint64_t n, uint32_t nk, const ggms_opts& o, int cus) {
   Config c{};
-  c.B = o.num_buckets ? o.num_buckets : 8192;
-  c.B = std::max<uint32_t>(16, std::min<uint32_t>(c.B, kMaxBuckets));
-  c.B_cap = kMaxBuckets;
   c.mode = kModeLinear;
   if (alg == GGMS_ALG_DISTRIBUTED_1GPU) c.mode = kModeStrided;
   if (o.splitter == GGMS_SPLIT_LEARNED) c.mode = kModeLearned;
   if (o.splitter == GGMS_SPLIT_SKETCH) c.mode = kModeSketch;
+  // 0 = automatic: the reference's 8192 for the linear splitter; 16384 for the
+  // sample-quantile splitters, whose near equal-depth buckets then hand most
+  // k-buckets straight to the finishing kernel
+  c.B = o.num_buckets ? o.num_buckets : ((c.mode == kModeLinear || c.mode == kModeStrided) ? 8192u : 16384u);
+  c.B = std::max<uint32_t>(16, std::min<uint32_t>(c.B, kMaxBuckets));
+  c.B_cap = kMaxBuckets;
   if (c.mode == kModeLinear || c.mode == kModeStrided) {
     c.P = o.num_pivots ? o.num_pivots : 17;
     c.P = std::max<uint32_t>(2, std::min<uint32_t>(c.P, 1024));
@@ -104,7 +107,7 @@ static Config resolve_config(ggms_alg alg, uint64_t n, uint32_t nk, const ggms_o
   c.min_tile = 16384;
   c.max_passes = o.max_passes ? std::min<uint32_t>(o.max_passes, GGMS_MAX_PASS_STATS) : 64;
   if (c.max_passes < 2) c.max_passes = 2;  // the first pass always hands its k-buckets to a second
-  c.fin_cap = sizeof(K) == 4 ? 16384 : 8192;
+  c.fin_cap = sizeof(K) == 4 ? 32768 : 16384;  // 128 KB of LDS in the finishing kernel
   c.seed = o.seed;
   c.force_finish = 0;
   (void)n;
@@ -245,7 +248,7 @@ static Layout make_layout(uint64_t n, uint32_t nk, const Config& c) {
   sz[bSmallKFill] = Bm * 4;
   sz[bWgFirst] = (size_t)c.G_max * 4 * 2;  // double-buffered by pass parity
   sz[bOut] = nkp * 8;
-  sz[bLut] = (size_t)lut_cells<K>() * sizeof(LutCell<K>);
+  sz[bLut] = LutGeom<K>::bytes();
   sz[bParams1] = Bm * sizeof(MapParam<K>);
   sz[bSmallKlo] = Bm * sizeof(K);
   sz[bSmallKhi] = Bm * sizeof(K);
@@ -385,11 +388,11 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
     pa.strideS = 8192;
     pa.seed = c.seed;
     pa.mlp = c.mode == kModeLearned ? learned_weights_device(ctx->device) : nullptr;
-    size_t lds_p0 = (size_t)pow2ceil_host(c.S) * sizeof(K) + 2048 * sizeof(K);
+    size_t lds_p0 = plan0_lds_bytes<K>(pow2ceil_host(c.S));
     LDS_CHECK(lds_p0);
     set_lds_attr((const void*)k_plan0<T>, lds_p0);
     prof.begin(GGMS_KERNEL_PLAN0, 0);
-    k_plan0<T><<<1, 1024, lds_p0, st>>>(d_vec, pa, rng_first, (LutCell<K>*)P(bLut), ctl);
+    k_plan0<T><<<1, 1024, lds_p0, st>>>(d_vec, pa, rng_first, (K*)P(bLut), ctl);
     prof.end();
     HIP_TRY(hipGetLastError());
 
@@ -424,14 +427,14 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
         LDS_CHECK(lds);
         set_lds_attr((const void*)k_count0<T>, lds);
         prof.begin(GGMS_KERNEL_COUNT0, (int)pass);
-        k_count0<T><<<G, NT, lds, st>>>(d_vec, rc, (LutCell<K>*)P(bLut), cp, rows);
+        k_count0<T><<<G, NT, lds, st>>>(d_vec, rc, (K*)P(bLut), cp, rows);
         prof.end();
       } else if (pass == 1) {
         size_t lds = ((size_t)B_bound * 4 + 15) / 16 * 16 + pass1_map_bytes<K>(B0, R0, p1_lds ? R_bound : 0);
         LDS_CHECK(lds);
         set_lds_attr((const void*)k_count1<K>, lds);
         prof.begin(GGMS_KERNEL_COUNTS, (int)pass);
-        k_count1<K><<<G, NT, lds, st>>>(src, ctl, (LutCell<K>*)P(bLut), rng_first, slot_tab[0],
+        k_count1<K><<<G, NT, lds, st>>>(src, ctl, (K*)P(bLut), rng_first, slot_tab[0],
                                         (MapParam<K>*)P(bParams1), p1_lds, cp, rows);
         prof.end();
       } else {
@@ -497,7 +500,7 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
         LDS_CHECK(lds);
         set_lds_attr((const void*)k_compact0<T>, lds);
         prof.begin(GGMS_KERNEL_COMPACT0, (int)pass);
-        k_compact0<T><<<G, NT, lds, st>>>(d_vec, rc, (LutCell<K>*)P(bLut), cp, rows, pl.slot_of_bucket, pl.slot_bucket,
+        k_compact0<T><<<G, NT, lds, st>>>(d_vec, rc, (K*)P(bLut), cp, rows, pl.slot_of_bucket, pl.slot_bucket,
                                           dst);
         prof.end();
       } else if (pass == 1) {
@@ -505,7 +508,7 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
         LDS_CHECK(lds);
         set_lds_attr((const void*)k_compact1<K>, lds);
         prof.begin(GGMS_KERNEL_COMPACTS, (int)pass);
-        k_compact1<K><<<G, NT, lds, st>>>(src, ctl, (LutCell<K>*)P(bLut), rng_first, slot_tab[0],
+        k_compact1<K><<<G, NT, lds, st>>>(src, ctl, (K*)P(bLut), rng_first, slot_tab[0],
                                           (MapParam<K>*)P(bParams1), p1_lds, cp, rows,
                                           pl.slot_of_bucket, pl.slot_dst, pl.slot_bucket, dst);
         prof.end();
@@ -612,15 +615,15 @@ static ggms_status run_first_pass(ggms_ctx* ctx, const T* d_vec, uint64_t n64, c
   pa.strideS = 8192;
   pa.seed = c.seed;
   pa.mlp = c.mode == kModeLearned ? learned_weights_device(ctx->device) : nullptr;
-  size_t lds_p0 = (size_t)pow2ceil_host(c.S) * sizeof(K) + 2048 * sizeof(K);
+  size_t lds_p0 = plan0_lds_bytes<K>(pow2ceil_host(c.S));
   LDS_CHECK(lds_p0);
   set_lds_attr((const void*)k_plan0<T>, lds_p0);
-  k_plan0<T><<<1, 1024, lds_p0, st>>>(d_vec, pa, rng, (LutCell<K>*)P(bLut), ctl);
+  k_plan0<T><<<1, 1024, lds_p0, st>>>(d_vec, pa, rng, (K*)P(bLut), ctl);
   HIP_TRY(hipGetLastError());
   size_t lds = count_lds0<T>(c);
   LDS_CHECK(lds);
   set_lds_attr((const void*)k_count0<T>, lds);
-  k_count0<T><<<G0, c.NT, lds, st>>>(d_vec, rng, (LutCell<K>*)P(bLut), ctl, rows);
+  k_count0<T><<<G0, c.NT, lds, st>>>(d_vec, rng, (K*)P(bLut), ctl, rows);
   HIP_TRY(hipGetLastError());
   k_rowscan<<<(c.B0_bound + 63) / 64, 1024, 0, st>>>(rows, ctl, totals);
   HIP_TRY(hipGetLastError());
@@ -691,7 +694,7 @@ extern "C" {
 void ggms_opts_default(ggms_opts* o) {
   if (!o) return;
   memset(o, 0, sizeof(*o));
-  o->num_buckets = 8192;
+  o->num_buckets = 0;
   o->num_pivots = 17;
   o->sample_size = 0;
   o->block_threads = 1024;

@@ -226,21 +226,27 @@ __device__ __forceinline__ void stream_tile_any(const V* __restrict__ x, uint32_
 }
 
 // ---------------------------------------------------------------------------
-// First-pass map in LDS.  A 4096-entry table indexed by the top 12 key bits
-// describes every key-space cell: its first range r0 and, when exactly one
-// range starts inside the cell, that start's offset from the cell's first key.
-// The range of a key is then  r0 + (low bits of key >= offset)  with no
-// branch; only cells holding two or more range starts (flagged) fall back to
-// a short binary search, guarded by a wave-uniform test.  The map parameters
-// of a range are packed for one 16-byte (u32 keys) or 32-byte (u64 keys) LDS
-// read.
+// First-pass map in LDS: a two-level key-prefix table.
+//
+// Level 1 has 4096 cells (top 12 key bits).  A cell whose key span contains at
+// most one range start is a leaf: its first range r0 and the offset of that
+// start inside the cell, so the range of a key is
+//     r0 + ((key & cell_mask) > start_offset - 1)
+// with no branch.  A cell holding two or more starts points to a level-2
+// table of 64 sub-cells (next 6 key bits) in the same leaf format; when the
+// level-2 budget is used up, or a sub-cell still holds two or more starts,
+// the entry says "search" and carries the first/last candidate range for a
+// short binary search.  Both rarer paths sit behind wave-uniform tests.
+//
+// Entry layout (E = K: 32-bit entries for u32 keys, 64-bit for u64 keys;
+// KB = key bits, S1 = KB - 12, S2 = S1 - 6):
+//   level-1 leaf:   bit KB-1 = 0 | r0 in bits [S1, S1+11) | start_offset-1 in bits [0, S1)
+//   level-2 leaf:   bit KB-1 = 0 | r0 in bits [S2, S2+17) | start_offset-1 in bits [0, S2)
+//   level-1 table:  bit KB-1 = 1, bit KB-2 = 1 | table index
+//   search (1, 2):  bit KB-1 = 1, bit KB-2 = 0 | r0 in bits [0, 11) | r1 in bits [11, 22)
+// The map parameters of a range are packed for one 16-byte (u32 keys) or
+// 32-byte (u64 keys) LDS read.
 // ---------------------------------------------------------------------------
-// 4096 cells of 8 bytes for u32 keys, 2048 cells of 16 bytes for u64 keys
-// (32 KB of LDS either way).
-template <class K> __host__ __device__ __forceinline__ constexpr uint32_t lut_bits() { return sizeof(K) == 4 ? 12u : 11u; }
-template <class K> __host__ __device__ __forceinline__ constexpr uint32_t lut_cells() { return 1u << lut_bits<K>(); }
-constexpr uint32_t kLutMulti = 0x80000000u;
-
 template <class K> struct MapParam;
 template <> struct __align__(16) MapParam<uint32_t> {
   uint32_t lo, base, mult, sh;
@@ -250,18 +256,20 @@ template <> struct __align__(16) MapParam<uint64_t> {
   uint32_t base, mult, sh, pad0, pad1, pad2;
 };
 
-template <class K> struct LutCell;
-template <> struct __align__(8) LutCell<uint32_t> {
-  uint32_t boff;  // offset of the single range start inside the cell, or the cell size
-  uint32_t r;     // r0 | r1 << 15 | kLutMulti when >= 2 starts
+template <class K> struct LutGeom {
+  static constexpr uint32_t KB = sizeof(K) * 8;
+  static constexpr uint32_t L1BITS = 12;
+  static constexpr uint32_t L1CELLS = 1u << L1BITS;
+  static constexpr uint32_t S1 = KB - L1BITS;
+  static constexpr uint32_t S2 = S1 - 6;
+  static constexpr uint32_t L2TABLES = sizeof(K) == 4 ? 64u : 32u;  // 16 KB of level-2 either way
+  static constexpr K FLAG = (K)1 << (KB - 1);
+  static constexpr K TABLE = (K)1 << (KB - 2);
+  static constexpr K M1 = ((K)1 << S1) - 1;
+  static constexpr K M2 = ((K)1 << S2) - 1;
+  static __host__ __device__ constexpr size_t bytes() { return (size_t)(L1CELLS + L2TABLES * 64) * sizeof(K); }
+  static __host__ __device__ constexpr uint32_t max_ranges() { return 2047; }
 };
-template <> struct __align__(16) LutCell<uint64_t> {
-  uint64_t boff;
-  uint32_t r, pad;
-};
-
-template <class K> __host__ __device__ __forceinline__ constexpr uint32_t lut_shift() { return sizeof(K) * 8 - lut_bits<K>(); }
-template <class K> __device__ __forceinline__ uint32_t lut_cell(K key) { return (uint32_t)(key >> lut_shift<K>()); }
 
 template <class K> __device__ __forceinline__ MapParam<K> to_param(const Range<K>& r) {
   MapParam<K> q{};
@@ -280,33 +288,37 @@ template <class K> __device__ __forceinline__ uint32_t param_bucket(const MapPar
 }
 
 template <class K> struct Pass0Map {
-  LutCell<K>* lut;
+  using G = LutGeom<K>;
+  K* lut;  // level 1 followed by the level-2 tables
   MapParam<K>* prm;
 
-  static __host__ __device__ size_t bytes(uint32_t R) {
-    return (size_t)lut_cells<K>() * sizeof(LutCell<K>) + (size_t)R * sizeof(MapParam<K>);
-  }
+  static __host__ __device__ size_t bytes(uint32_t R) { return G::bytes() + (size_t)R * sizeof(MapParam<K>); }
   __device__ void carve(unsigned char* p) {
-    lut = reinterpret_cast<LutCell<K>*>(p);
-    prm = reinterpret_cast<MapParam<K>*>(p + (size_t)lut_cells<K>() * sizeof(LutCell<K>));
+    lut = reinterpret_cast<K*>(p);
+    prm = reinterpret_cast<MapParam<K>*>(p + G::bytes());
   }
-  __device__ void load(const LutCell<K>* __restrict__ g_lut, const Range<K>* __restrict__ g, uint32_t R) {
-    for (uint32_t i = threadIdx.x; i < lut_cells<K>(); i += blockDim.x) lut[i] = g_lut[i];
+  __device__ void load(const K* __restrict__ g_lut, const Range<K>* __restrict__ g, uint32_t R) {
+    for (uint32_t i = threadIdx.x; i < G::L1CELLS + G::L2TABLES * 64; i += blockDim.x) lut[i] = g_lut[i];
     for (uint32_t i = threadIdx.x; i < R; i += blockDim.x) prm[i] = to_param<K>(g[i]);
   }
   __device__ __forceinline__ uint32_t range_of(K key) const {
-    const LutCell<K> e = lut[lut_cell<K>(key)];
-    const K low = key & (K)(((K)1 << lut_shift<K>()) - 1);
-    uint32_t r = (e.r & 0x7FFFu) + (low >= e.boff ? 1u : 0u);
-    bool multi = (e.r & kLutMulti) != 0;
-    if (wave_ballot(multi)) {
-      if (multi) {
-        uint32_t lo = e.r & 0x7FFFu, hi = (e.r >> 15) & 0x7FFFu;
-        while (lo < hi) {
-          uint32_t mid = (lo + hi + 1) >> 1;
-          if (prm[mid].lo <= key) lo = mid; else hi = mid - 1;
+    K e = lut[(uint32_t)(key >> G::S1)];
+    uint32_t r = (uint32_t)((e >> G::S1) & 0x7FF) + ((key & G::M1) > (e & G::M1) ? 1u : 0u);
+    bool nl = (e & G::FLAG) != 0;
+    if (wave_ballot(nl)) {
+      if (nl) {
+        if (e & G::TABLE) {
+          e = lut[G::L1CELLS + (((uint32_t)(e & (G::TABLE - 1))) << 6) + ((uint32_t)(key >> G::S2) & 63u)];
+          r = (uint32_t)((e >> G::S2) & 0x1FFFF) + ((key & G::M2) > (e & G::M2) ? 1u : 0u);
         }
-        r = lo;
+        if (e & G::FLAG) {
+          uint32_t lo = (uint32_t)(e & 0x7FF), hi = (uint32_t)((e >> 11) & 0x7FF);
+          while (lo < hi) {
+            uint32_t mid = (lo + hi + 1) >> 1;
+            if (prm[mid].lo <= key) lo = mid; else hi = mid - 1;
+          }
+          r = lo;
+        }
       }
     }
     return r;
@@ -314,42 +326,68 @@ template <class K> struct Pass0Map {
   __device__ __forceinline__ uint32_t bucket(K key) const { return param_bucket<K>(prm[range_of(key)], key); }
 };
 
-// Cell table of a range list (ranges partition the key space, sorted by lo).
+// Range of the last start <= key among starts[lo..hi].
+template <class K> __device__ __forceinline__ uint32_t last_start_le(const K* starts, uint32_t lo, uint32_t hi, K key) {
+  while (lo < hi) {
+    uint32_t mid = (lo + hi + 1) >> 1;
+    if (starts[mid] <= key) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Two-level table of a range list given its sorted starts (in LDS; the ranges
+// partition the key space, starts[0] == 0, R <= 2047).  Called by a whole
+// workgroup; writes level 1 and the level-2 tables to g_lut.
 template <class K>
-__device__ void build_lut(const Range<K>* __restrict__ ranges, uint32_t R, LutCell<K>* __restrict__ g_lut) {
-  constexpr uint32_t sh = lut_shift<K>();
-  for (uint32_t c = threadIdx.x; c < lut_cells<K>(); c += blockDim.x) {
-    K c0 = (K)((K)c << sh);
-    K c1 = (K)(c0 + (((K)1 << sh) - 1));
-    uint32_t r0, r1;
-    {
-      uint32_t lo = 0, hi = R - 1;
-      while (lo < hi) {
-        uint32_t mid = (lo + hi + 1) >> 1;
-        if (ranges[mid].lo <= c0) lo = mid; else hi = mid - 1;
-      }
-      r0 = lo;
-    }
-    {
-      uint32_t lo = r0, hi = R - 1;
-      while (lo < hi) {
-        uint32_t mid = (lo + hi + 1) >> 1;
-        if (ranges[mid].lo <= c1) lo = mid; else hi = mid - 1;
-      }
-      r1 = lo;
-    }
-    LutCell<K> e{};
+__device__ void build_lut(const K* starts, uint32_t R, K* __restrict__ g_lut) {
+  using G = LutGeom<K>;
+  __shared__ uint32_t tab_cell[G::L2TABLES];
+  __shared__ uint32_t scan_s[32];
+  const uint32_t nt = blockDim.x;
+  // level 1; multi-start cells are numbered in cell order for level-2 tables
+  uint32_t per = (G::L1CELLS + nt - 1) / nt;
+  uint32_t c0 = threadIdx.x * per, c1 = c0 + per < G::L1CELLS ? c0 + per : G::L1CELLS;
+  uint32_t nmulti = 0;
+  for (uint32_t c = c0; c < c1; ++c) {
+    K lo = (K)((K)c << G::S1), hi = (K)(lo + G::M1);
+    uint32_t r0 = last_start_le<K>(starts, 0, R - 1, lo);
+    uint32_t r1 = last_start_le<K>(starts, r0, R - 1, hi);
+    nmulti += r1 > r0 + 1;
+  }
+  uint32_t tot;
+  uint32_t t = block_excl_scan_u32(nmulti, scan_s, &tot);
+  for (uint32_t c = c0; c < c1; ++c) {
+    K lo = (K)((K)c << G::S1), hi = (K)(lo + G::M1);
+    uint32_t r0 = last_start_le<K>(starts, 0, R - 1, lo);
+    uint32_t r1 = last_start_le<K>(starts, r0, R - 1, hi);
+    K e;
     if (r1 == r0) {
-      e.boff = (K)((K)1 << sh);  // no start inside: the compare never fires
-      e.r = r0;
+      e = ((K)r0 << G::S1) | G::M1;
     } else if (r1 == r0 + 1) {
-      e.boff = (K)(ranges[r1].lo - c0);
-      e.r = r0;
+      e = ((K)r0 << G::S1) | (K)(starts[r1] - lo - 1);
+    } else if (t < G::L2TABLES) {
+      tab_cell[t] = c;
+      e = G::FLAG | G::TABLE | (K)t;
+      t++;
     } else {
-      e.boff = (K)((K)1 << sh);
-      e.r = r0 | (r1 << 15) | kLutMulti;
+      e = G::FLAG | (K)r0 | ((K)r1 << 11);
     }
     g_lut[c] = e;
+  }
+  __syncthreads();
+  // level 2: 64 sub-cells of each tabled cell
+  uint32_t used = tot < G::L2TABLES ? tot : G::L2TABLES;
+  for (uint32_t i = threadIdx.x; i < used * 64; i += nt) {
+    uint32_t tb = i >> 6, sub = i & 63;
+    uint32_t c = tab_cell[tb];
+    K lo = (K)(((K)c << G::S1) | ((K)sub << G::S2)), hi = (K)(lo + G::M2);
+    uint32_t r0 = last_start_le<K>(starts, 0, R - 1, lo);
+    uint32_t r1 = last_start_le<K>(starts, r0, R - 1, hi);
+    K e;
+    if (r1 == r0) e = ((K)r0 << G::S2) | G::M2;
+    else if (r1 == r0 + 1) e = ((K)r0 << G::S2) | (K)(starts[r1] - lo - 1);
+    else e = G::FLAG | (K)r0 | ((K)r1 << 11);
+    g_lut[G::L1CELLS + i] = e;
   }
 }
 
@@ -382,7 +420,7 @@ __global__ void k_init_ks(const uint32_t* __restrict__ ks, uint32_t nk, uint32_t
 // ---------------------------------------------------------------------------
 template <class T>
 __global__ void __launch_bounds__(1024) k_count0(const T* __restrict__ x, const Range<typename KeyTraits<T>::K>* __restrict__ ranges,
-                                                 const LutCell<typename KeyTraits<T>::K>* __restrict__ lut,
+                                                 const typename KeyTraits<T>::K* __restrict__ lut,
                                                  const PassCtl* __restrict__ ctl, uint32_t* __restrict__ rows) {
   using K = typename KeyTraits<T>::K;
   extern __shared__ __align__(16) unsigned char smem[];
@@ -551,8 +589,12 @@ __global__ void __launch_bounds__(1024) k_plan(PlanArgs<typename KeyTraits<T>::K
     }
     aidx[lo] = 1;
     a.k_aux[k] = lo;
+    a.k_rank[k] = g - (P[lo] - a.totals[lo]);  // rank inside the bucket (1-based)
   }
   __syncthreads();
+  // P is free from here on: reuse it for the ranges' first buckets
+  uint32_t* rbase = P;
+  for (uint32_t r = t; r < ctl.R; r += nt) rbase[r] = a.ranges[r].base;
 
   // P2: number the active buckets in key order
   {
@@ -579,7 +621,7 @@ __global__ void __launch_bounds__(1024) k_plan(PlanArgs<typename KeyTraits<T>::K
     uint32_t lo = 0, hi = ctl.R - 1;
     while (lo < hi) {
       uint32_t mid = (lo + hi + 1) >> 1;
-      if (a.ranges[mid].base <= b) lo = mid; else hi = mid - 1;
+      if (rbase[mid] <= b) lo = mid; else hi = mid - 1;
     }
     Range<K> rg = a.ranges[lo];
     K klo = 0, khi = 0;
@@ -700,8 +742,7 @@ __global__ void __launch_bounds__(1024) k_plan(PlanArgs<typename KeyTraits<T>::K
     if (a.k_state[k] != kLive) continue;
     uint32_t b = a.k_aux[k];
     uint32_t q = aidx[b] - 1;
-    uint32_t c = a.totals[b];
-    uint32_t local = a.k_rank[k] - (P[b] - c);
+    uint32_t local = a.k_rank[k];
     uint32_t cls = a.act_cls[q];
     if (cls == kResolved) {
       out[k] = KeyTraits<T>::dec(a.act_klo[q]);
@@ -827,7 +868,7 @@ __host__ __device__ __forceinline__ size_t compact_lds_bytes(uint32_t B, uint32_
 // per survivor.  The next pass finds each element's range from its key.
 template <class T>
 __global__ void __launch_bounds__(1024) k_compact0(const T* __restrict__ x, const Range<typename KeyTraits<T>::K>* __restrict__ ranges,
-                                                   const LutCell<typename KeyTraits<T>::K>* __restrict__ lut,
+                                                   const typename KeyTraits<T>::K* __restrict__ lut,
                                                    const PassCtl* __restrict__ ctl, const uint32_t* __restrict__ rows,
                                                    const uint16_t* __restrict__ slot_of_bucket,
                                                    const uint32_t* __restrict__ slot_bucket,
@@ -890,7 +931,7 @@ __host__ __device__ __forceinline__ size_t pass1_map_bytes(uint32_t B0, uint32_t
 
 template <class K>
 __device__ __forceinline__ void pass1_map_init(Pass1Map<K>& pm, unsigned char* p, const PassCtl* ctl0,
-                                               const LutCell<K>* lut, const Range<K>* rng0, const uint16_t* slot0_g,
+                                               const K* lut, const Range<K>* rng0, const uint16_t* slot0_g,
                                                const MapParam<K>* p1_g, uint32_t R1, bool p1_in_lds) {
   const uint32_t B0 = ctl0->B, R0 = ctl0->R;
   uint16_t* s0 = reinterpret_cast<uint16_t*>(p);
@@ -910,7 +951,7 @@ __device__ __forceinline__ void pass1_map_init(Pass1Map<K>& pm, unsigned char* p
 
 template <class K>
 __global__ void __launch_bounds__(1024) k_count1(const K* __restrict__ buf, const PassCtl* __restrict__ ctl0,
-                                                 const LutCell<K>* __restrict__ lut, const Range<K>* __restrict__ rng0,
+                                                 const K* __restrict__ lut, const Range<K>* __restrict__ rng0,
                                                  const uint16_t* __restrict__ slot0, const MapParam<K>* __restrict__ p1,
                                                  uint32_t p1_in_lds, const PassCtl* __restrict__ ctl,
                                                  uint32_t* __restrict__ rows) {
@@ -932,7 +973,7 @@ __global__ void __launch_bounds__(1024) k_count1(const K* __restrict__ buf, cons
 
 template <class K>
 __global__ void __launch_bounds__(1024) k_compact1(const K* __restrict__ buf, const PassCtl* __restrict__ ctl0,
-                                                   const LutCell<K>* __restrict__ lut, const Range<K>* __restrict__ rng0,
+                                                   const K* __restrict__ lut, const Range<K>* __restrict__ rng0,
                                                    const uint16_t* __restrict__ slot0, const MapParam<K>* __restrict__ p1,
                                                    uint32_t p1_in_lds, const PassCtl* __restrict__ ctl,
                                                    const uint32_t* __restrict__ rows,

@@ -30,6 +30,14 @@ namespace ggms {
 constexpr int kMlpIn = 65, kMlpHidden = 32, kMlpWin = 64;
 constexpr int kMlpParams = kMlpHidden * kMlpIn + kMlpHidden + kMlpHidden + 1;
 
+// LDS of k_plan0: sorted sample, pivots (<= 2048), range starts, requested and
+// final bucket counts (at most 2 * 1025 + 6 ranges).
+constexpr uint32_t kPlan0MaxRanges = 2064;
+template <class K> __host__ __device__ constexpr size_t plan0_lds_bytes(uint32_t S_pad) {
+  return (size_t)S_pad * sizeof(K) + 2048 * sizeof(K) + kPlan0MaxRanges * sizeof(K) +
+         (kPlan0MaxRanges * 8 > kMlpParams * 4 ? kPlan0MaxRanges * 8 : kMlpParams * 4);
+}
+
 template <class K>
 __device__ __forceinline__ double key_to_double(K k);
 template <>
@@ -44,7 +52,7 @@ __device__ __forceinline__ double key_to_double<uint64_t>(uint64_t k) { return (
 // the true quantile lies (sigmoid output).  The prediction therefore always
 // falls between those two sample keys.
 template <class K>
-__device__ K learned_pivot(const K* s, uint32_t S, double rank_frac, const float* __restrict__ w) {
+__device__ K learned_pivot(const K* s, uint32_t S, double rank_frac, const float* w) {
   double pos = rank_frac * (double)(S - 1);
   int ip = (int)pos;
   if (ip >= (int)S - 1) ip = (int)S - 2;
@@ -80,7 +88,7 @@ __device__ K learned_pivot(const K* s, uint32_t S, double rank_frac, const float
 template <class T>
 __global__ void __launch_bounds__(1024) k_plan0(const T* __restrict__ x, Plan0Args a,
                                                 Range<typename KeyTraits<T>::K>* __restrict__ ranges,
-                                                LutCell<typename KeyTraits<T>::K>* __restrict__ lut,
+                                                typename KeyTraits<T>::K* __restrict__ lut,
                                                 PassCtl* __restrict__ ctl) {
   using K = typename KeyTraits<T>::K;
   using KT = KeyTraits<T>;
@@ -90,7 +98,7 @@ __global__ void __launch_bounds__(1024) k_plan0(const T* __restrict__ x, Plan0Ar
   const uint32_t S = a.S;
   const uint32_t Spad = pow2ceil_u32(S);
   K* s = reinterpret_cast<K*>(smem);
-  K* q = s + Spad;  // pivot keys, up to 2048
+  K* q = s + Spad;  // pivot keys, up to 2048, then the range list
   const K lowest = KT::kFloat ? KT::kNegInf : (K)0;
   const K highest = KT::kFloat ? KT::kPosInf : KeyMax<K>::v;
 
@@ -145,11 +153,18 @@ __global__ void __launch_bounds__(1024) k_plan0(const T* __restrict__ x, Plan0Ar
   } else {
     const uint32_t Pl = a.P;  // pieces
     Q = Pl + 1;
+    // network weights staged in LDS (every thread reads the same weight at the
+    // same time: broadcast reads)
+    float* wl = reinterpret_cast<float*>(q + 2048 + kPlan0MaxRanges);
+    if (a.mode == kModeLearned) {
+      for (uint32_t i = threadIdx.x; i < (uint32_t)kMlpParams; i += blockDim.x) wl[i] = a.mlp[i];
+      __syncthreads();
+    }
     for (uint32_t j = threadIdx.x; j <= Pl; j += blockDim.x) {
       K v;
       if (j == 0) v = s[0];
       else if (j == Pl) v = s[S - 1];
-      else if (a.mode == kModeLearned) v = learned_pivot<K>(s, S, (double)j / (double)Pl, a.mlp);
+      else if (a.mode == kModeLearned) v = learned_pivot<K>(s, S, (double)j / (double)Pl, wl);
       else {
         double pos = (double)j / (double)Pl * (double)(S - 1);
         v = s[(uint32_t)(pos + 0.5)];
@@ -165,68 +180,90 @@ __global__ void __launch_bounds__(1024) k_plan0(const T* __restrict__ x, Plan0Ar
   }
   __syncthreads();
 
-  // 3. ranges (serial; at most a few hundred entries)
+  // 3. range list: starts and requested bucket counts, emitted in order by one
+  //    thread into LDS (cheap: no map arithmetic), then finished in parallel.
+  K* rlo = q + 2048;                                                  // [kPlan0MaxRanges]
+  // the network weights (only used before this point) share this space
+  uint32_t* rreq = reinterpret_cast<uint32_t*>(rlo + kPlan0MaxRanges);  // [kPlan0MaxRanges]
+  uint32_t* rnb = rreq + kPlan0MaxRanges;                             // [kPlan0MaxRanges]
   if (threadIdx.x == 0) {
     enum { kSingle = 0, kEdge = 1, kInterior = 2, kNan = 3 };
     uint32_t m_int = 0;
-    uint32_t R = 0, base = 0;
-    uint32_t nan_lo = kNone32, nan_hi = kNone32;
+    uint32_t R = 0;
     uint32_t share = 1;
     for (int emit = 0; emit < 2; ++emit) {
-      auto cb = [&](K lo, K hi, int kind) {
+      auto cb = [&](K lo, int kind) {
         if (!emit) {
           if (kind == kInterior) m_int++;
           return;
         }
-        uint32_t req = kind == kInterior ? share : kind == kEdge ? a.E : 1u;
-        Range<K> r;
-        r.lo = lo;
-        r.hi = hi;
-        r.seg_start = 0;
-        r.seg_end = 0;
-        make_map<K>(r, req);
-        r.base = base;
-        if (kind == kNan) {
-          if (R == 0) nan_lo = base; else nan_hi = base;
-        }
-        base += r.nb;
-        ranges[R++] = r;
+        rlo[R] = lo;
+        rreq[R] = kind == kInterior ? share : kind == kEdge ? a.E : 1u;
+        R++;
       };
-      if (KT::kFloat) cb((K)0, (K)(lowest - 1), kNan);
-      if (q[0] > lowest) cb(lowest, (K)(q[0] - 1), kEdge);
+      if (KT::kFloat) cb((K)0, kNan);
+      if (q[0] > lowest) cb(lowest, kEdge);
       uint32_t i = 0;
       while (i < Q) {
         K v = q[i];
         uint32_t j = i;
         while (j + 1 < Q && q[j + 1] == v) ++j;
         if (j + 1 >= Q) {
-          cb(v, v, kSingle);
+          cb(v, kSingle);
           break;
         }
         K nxt = q[j + 1];
         if (j > i) {
-          cb(v, v, kSingle);
-          if ((K)(nxt - v) >= 2) cb((K)(v + 1), (K)(nxt - 1), kInterior);
+          cb(v, kSingle);
+          if ((K)(nxt - v) >= 2) cb((K)(v + 1), kInterior);
         } else {
-          cb(v, (K)(nxt - 1), kInterior);
+          cb(v, kInterior);
         }
         i = j + 1;
       }
-      if (q[Q - 1] < highest) cb((K)(q[Q - 1] + 1), highest, kEdge);
-      if (KT::kFloat) cb((K)(highest + 1), KeyMax<K>::v, kNan);
+      if (q[Q - 1] < highest) cb((K)(q[Q - 1] + 1), kEdge);
+      if (KT::kFloat) cb((K)(highest + 1), kNan);
       if (!emit) share = m_int ? (a.B / m_int > 0 ? a.B / m_int : 1) : 1;
     }
-    ctl->R = R;
-    ctl->Rp = pow2ceil_u32(R);
-    ctl->B = base;
-    ctl->nan_lo_bucket = nan_lo;
-    ctl->nan_hi_bucket = nan_hi;
     bcast_R = R;
   }
-  __threadfence_block();
   __syncthreads();
-  // 4. key-prefix cell table of the ranges
-  build_lut<K>(ranges, bcast_R, lut);
+  const uint32_t R = bcast_R;
+  // 4. map parameters (parallel), bucket bases (block scan), global copy
+  __shared__ uint32_t scan_tmp[32];
+  uint32_t per = (R + blockDim.x - 1) / blockDim.x;
+  uint32_t r0 = threadIdx.x * per, r1 = r0 + per < R ? r0 + per : R;
+  uint32_t mysum = 0;
+  for (uint32_t r = r0; r < r1; ++r) {
+    Range<K> rr;
+    rr.lo = rlo[r];
+    rr.hi = r + 1 < R ? (K)(rlo[r + 1] - 1) : KeyMax<K>::v;
+    make_map<K>(rr, rreq[r]);
+    rnb[r] = rr.nb;
+    mysum += rr.nb;
+  }
+  uint32_t Btot;
+  uint32_t off = block_excl_scan_u32(mysum, scan_tmp, &Btot);
+  for (uint32_t r = r0; r < r1; ++r) {
+    Range<K> rr;
+    rr.lo = rlo[r];
+    rr.hi = r + 1 < R ? (K)(rlo[r + 1] - 1) : KeyMax<K>::v;
+    make_map<K>(rr, rreq[r]);
+    rr.base = off;
+    rr.seg_start = 0;
+    rr.seg_end = 0;
+    off += rr.nb;
+    ranges[r] = rr;
+  }
+  if (threadIdx.x == 0) {
+    ctl->R = R;
+    ctl->Rp = pow2ceil_u32(R);
+    ctl->B = Btot;
+    ctl->nan_lo_bucket = KT::kFloat ? 0u : kNone32;
+    ctl->nan_hi_bucket = KT::kFloat ? Btot - 1 : kNone32;
+  }
+  // 5. key-prefix cell table, from the LDS copy of the range starts
+  build_lut<K>(rlo, R, lut);
 }
 
 }  // namespace ggms

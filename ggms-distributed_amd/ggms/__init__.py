@@ -111,6 +111,8 @@ class Context:
         ks_np = np.ascontiguousarray(np.asarray(ks, dtype=np.uint32))
         out = np.empty(len(ks_np), dtype=_NP[dt])
         o = opts if opts is not None else default_opts()
+        if o.splitter == SPLITTERS["learned"]:
+            _ensure_learned(self.device)
         st = self.lib.ggms_select(self.h, ALGS[alg], dt, C.c_void_p(x.data_ptr()), n,
                                   ks_np.ctypes.data_as(_lib._U32P), len(ks_np), out.ctypes.data_as(C.c_void_p),
                                   C.byref(o))
@@ -121,6 +123,8 @@ class Context:
         """Device-resident ks (uint32 tensor) and outputs (tensor of x's dtype)."""
         dt = _dtype_code(x.dtype)
         o = opts if opts is not None else default_opts()
+        if o.splitter == SPLITTERS["learned"]:
+            _ensure_learned(self.device)
         st = self.lib.ggms_select_device(self.h, ALGS[alg], dt, C.c_void_p(x.data_ptr()), x.numel(),
                                          C.c_void_p(d_ks.data_ptr()), d_ks.numel(), C.c_void_p(d_out.data_ptr()),
                                          C.byref(o))
@@ -142,6 +146,24 @@ class Context:
                                            ("sh", "<u4"), ("mult", "<u4")])
         C.memmove(ranges.ctypes.data, rr, nr.value * C.sizeof(RangeU64))
         return ranges, hist[: nb.value].copy()
+
+
+_learned_installed: set = set()
+
+
+def _ensure_learned(device: int):
+    """Install the committed learned-splitter weights on first use (if present)."""
+    if device in _learned_installed:
+        return
+    from . import learned
+    if learned.install(device):
+        _learned_installed.add(device)
+
+
+def set_learned_weights(w, device: int = 0):
+    from . import learned
+    learned.install(device, w=np.asarray(w, dtype=np.float32))
+    _learned_installed.add(device)
 
 
 _default_ctx: dict = {}
@@ -198,6 +220,7 @@ def generate(dtype: str, dist: str, n: int, seed: int, p0: float = 0.0, p1: floa
     x = torch.empty(n, dtype=tt, device=f"cuda:{device}")
     st = _lib.load().ggms_generate(DTYPES[dtype], DISTS[dist], C.c_void_p(x.data_ptr()), n, seed, p0, p1, None)
     _check(st, "ggms_generate")
+    torch.cuda.synchronize(x.device)
     return x
 
 

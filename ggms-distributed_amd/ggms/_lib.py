@@ -77,6 +77,13 @@ SIGNATURES = {
 SIGNATURES["ggms_generate"] = (C.c_int, [C.c_int, C.c_int, _P, C.c_uint64, C.c_uint64, C.c_double, C.c_double, _P])
 SIGNATURES["ggms_mixture_params"] = (C.c_int, [C.c_uint64, C.POINTER(C.c_int), _P, _P])
 SIGNATURES["ggms_generate_ks"] = (C.c_int, [C.c_int, C.c_uint32, C.c_uint64, C.c_uint64, _U32P])
+SIGNATURES["ggms_mlp_param_count"] = (C.c_uint32, [])
+SIGNATURES["ggms_set_learned_weights"] = (C.c_int, [C.c_int, _P, C.c_uint32])
+SIGNATURES["ggms_clear_learned_weights"] = (C.c_int, [C.c_int])
+SIGNATURES["ggms_mlp_forward"] = (C.c_int, [_P, _P, C.c_uint32, _P, _P])
+SIGNATURES["ggms_mlp_workspace_floats"] = (C.c_uint64, [C.c_uint32])
+SIGNATURES["ggms_mlp_loss_grad"] = (C.c_int, [_P, _P, _P, C.c_uint32, _P, _P, _P, _P])
+SIGNATURES["ggms_mlp_adam_step"] = (C.c_int, [_P, _P, _P, _P, C.c_uint32, C.c_float, C.c_float, C.c_float, C.c_float, _P])
 for _alg in ("iterativeSMOSWrapper", "bucketMultiselectWrapper"):
     for _suf in ("f32", "f64", "u32", "i32"):
         SIGNATURES[f"ggms_{_alg}_{_suf}"] = (C.c_int, [_P, C.c_int, _U32P, C.c_int, _P, C.c_int, C.c_int])
@@ -94,6 +101,14 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         return _lib
     if not os.path.exists(path):
         raise RuntimeError(f"libggms.so not built at {path}; run ggms-distributed_amd/build.py")
+    # torch ships its own HIP runtime with the same soname (libamdhip64.so.7).
+    # Whichever is loaded first serves the whole process, so when torch is
+    # present it is loaded first: torch and this library then share one
+    # runtime (and one null stream), which is what torch needs to find the GPU.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(path)
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)

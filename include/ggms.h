@@ -52,7 +52,7 @@ typedef enum {
 
 /* Tunables.  Zero means "library default" for every numeric field. */
 typedef struct {
-  uint32_t num_buckets;   /* buckets per pass (default 8192, max 16384) */
+  uint32_t num_buckets;   /* buckets per pass (0 = auto: 8192 linear, 16384 learned/sketch; max 16384) */
   uint32_t num_pivots;    /* linear splitter pivots (default 17 => 16 ranges) */
   uint32_t sample_size;   /* splitter sample (default 1024; 8192 for the one-GPU distributed form) */
   uint32_t block_threads; /* workgroup size of the streaming kernels (default 1024) */
@@ -199,6 +199,30 @@ typedef enum {
 } ggms_kdist;
 
 ggms_status ggms_generate_ks(ggms_kdist kd, uint32_t nk, uint64_t n, uint64_t seed, uint32_t* ks_host);
+
+
+/* ------------------------------------------------------------------------
+ * Learned splitter (GGMS_SPLIT_LEARNED).  A 65 -> 32 (ReLU) -> 1 (sigmoid)
+ * fp32 MLP; parameters W1[32][65], b1[32], W2[32], b2 (2145 floats).  The
+ * first pass evaluates it on windows of the sorted sample to place its pivots
+ * (no reference counterpart: this is the project's extension of the
+ * reference splitter, generatePivots, DistributedSMOS/iterativeSMOS.cu:1011-1054).
+ * Training kernels operate on device arrays: x[nb][65], y[nb].
+ * ------------------------------------------------------------------------ */
+uint32_t ggms_mlp_param_count(void);
+/* Install the weights the learned splitter of `device` uses (host array). */
+ggms_status ggms_set_learned_weights(int device, const float* w_host, uint32_t n);
+ggms_status ggms_clear_learned_weights(int device);
+/* out[b] = sigmoid(MLP(x[b])) */
+ggms_status ggms_mlp_forward(const float* d_w, const float* d_x, uint32_t nb, float* d_out, void* stream);
+/* Workspace (floats) ggms_mlp_loss_grad needs for a batch of nb. */
+uint64_t ggms_mlp_workspace_floats(uint32_t nb);
+/* grad = d/dw of mean_b 0.5 (sigmoid(MLP(x_b)) - y_b)^2, loss = that mean. */
+ggms_status ggms_mlp_loss_grad(const float* d_w, const float* d_x, const float* d_y, uint32_t nb, float* d_grad,
+                               float* d_loss, float* d_work, void* stream);
+/* One Adam step (step counts from 1). */
+ggms_status ggms_mlp_adam_step(float* d_w, const float* d_grad, float* d_m, float* d_v, uint32_t step, float lr,
+                               float beta1, float beta2, float eps, void* stream);
 
 #ifdef __cplusplus
 }
