@@ -392,7 +392,7 @@ __device__ void build_lut(const K* starts, uint32_t R, K* __restrict__ g_lut) {
 }
 
 // Plain fill (used instead of hipMemsetAsync so the sequence stays capturable).
-__global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t words, uint32_t v) {
+static __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t words, uint32_t v) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < words) p[i] = v;
 }
@@ -400,7 +400,7 @@ __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t words, uint32_t v)
 // ---------------------------------------------------------------------------
 // k_init_ks: k-th largest (1-based) -> k-th smallest rank; validates range.
 // ---------------------------------------------------------------------------
-__global__ void k_init_ks(const uint32_t* __restrict__ ks, uint32_t nk, uint32_t n, uint32_t* __restrict__ k_rank,
+static __global__ void k_init_ks(const uint32_t* __restrict__ ks, uint32_t nk, uint32_t n, uint32_t* __restrict__ k_rank,
                           uint32_t* __restrict__ k_state, PassCtl* ctl0) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nk) return;
@@ -490,7 +490,7 @@ __global__ void __launch_bounds__(1024) k_countS(const K* __restrict__ buf, cons
 // k_rowscan: rows[G][B] -> exclusive prefix down each column (in place) and
 // the column totals.  64 columns per workgroup, waves split the rows.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(1024) k_rowscan(uint32_t* __restrict__ rows, const PassCtl* __restrict__ ctl,
+static __global__ void __launch_bounds__(1024) k_rowscan(uint32_t* __restrict__ rows, const PassCtl* __restrict__ ctl,
                                                   uint32_t* __restrict__ totals) {
   __shared__ uint32_t wsum[16][64];
   const uint32_t G = ctl->G, B = ctl->B;

@@ -59,6 +59,30 @@ class RangeU64(C.Structure):
     ]
 
 
+GGMS_BATCH_CLASSES = 24
+GGMS_BATCH_MAX_KS = 2048
+
+
+class BatchClass(C.Structure):
+    _fields_ = [
+        ("num_buckets", C.c_uint32),
+        ("sample_size", C.c_uint32),
+        ("pieces", C.c_uint32),
+        ("rebucket", C.c_uint32),
+    ]
+
+
+class BatchOpts(C.Structure):
+    _fields_ = [
+        ("workgroups", C.c_uint32),
+        ("block_threads", C.c_uint32),
+        ("use_graph", C.c_uint32),
+        ("reserved", C.c_uint32),
+        ("seed", C.c_uint64),
+        ("classes", BatchClass * GGMS_BATCH_CLASSES),
+    ]
+
+
 # (name, restype, argtypes) of every symbol include/ggms.h declares.
 _P = C.c_void_p
 _U32P = C.POINTER(C.c_uint32)
@@ -84,6 +108,15 @@ SIGNATURES["ggms_mlp_forward"] = (C.c_int, [_P, _P, C.c_uint32, _P, _P])
 SIGNATURES["ggms_mlp_workspace_floats"] = (C.c_uint64, [C.c_uint32])
 SIGNATURES["ggms_mlp_loss_grad"] = (C.c_int, [_P, _P, _P, C.c_uint32, _P, _P, _P, _P])
 SIGNATURES["ggms_mlp_adam_step"] = (C.c_int, [_P, _P, _P, _P, C.c_uint32, C.c_float, C.c_float, C.c_float, C.c_float, _P])
+SIGNATURES["ggms_batch_opts_default"] = (None, [C.POINTER(BatchOpts)])
+SIGNATURES["ggms_batch_default_classes"] = (None, [C.POINTER(BatchClass)])
+SIGNATURES["ggms_batch_plan_create"] = (C.c_int, [C.c_int, C.c_int, _P, C.c_uint32, C.POINTER(C.c_uint64), _P, _U32P,
+                                                  _P, C.POINTER(BatchOpts), C.POINTER(_P)])
+SIGNATURES["ggms_batch_plan_launch"] = (C.c_int, [_P, _P])
+SIGNATURES["ggms_batch_plan_status"] = (C.c_int, [_P, _U32P])
+SIGNATURES["ggms_batch_plan_destroy"] = (None, [_P])
+SIGNATURES["ggms_select_batched"] = (C.c_int, [C.c_int, C.c_int, _P, C.c_uint32, C.POINTER(C.c_uint64), _P, _U32P,
+                                               _P, C.POINTER(BatchOpts)])
 for _alg in ("iterativeSMOSWrapper", "bucketMultiselectWrapper"):
     for _suf in ("f32", "f64", "u32", "i32"):
         SIGNATURES[f"ggms_{_alg}_{_suf}"] = (C.c_int, [_P, C.c_int, _U32P, C.c_int, _P, C.c_int, C.c_int])

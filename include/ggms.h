@@ -165,6 +165,63 @@ ggms_status ggms_debug_first_pass(ggms_ctx* ctx, ggms_dtype t, const void* d_vec
                                   ggms_range_u64* ranges_host, uint32_t* num_ranges, uint32_t* hist_host,
                                   uint32_t* num_buckets);
 
+/* ------------------------------------------------------------------------
+ * Batched selection: many independent problems of uneven sizes solved by ONE
+ * persistent launch (no reference counterpart: the reference answers one
+ * problem per call, IterativeSMOS::iterativeSMOSWrapper,
+ * DistributedSMOS/iterativeSMOS.cu:1069-1577; each problem here has the same
+ * meaning as one such call).
+ *
+ * Problem p owns values[offsets[p] .. offsets[p+1]) (device array, one dtype
+ * for the batch), k ranks ks[koffsets[p] .. koffsets[p+1]) (device array,
+ * k = 1 is the largest, 1 <= k <= size, at most GGMS_BATCH_MAX_KS per
+ * problem) and writes out[koffsets[p] + j] = the ks[koffsets[p] + j]-th
+ * largest of its values.  offsets / koffsets are host arrays of
+ * num_problems + 1 entries.  Problem sizes are below 2^32.
+ *
+ * A plan fixes the buffers and shapes; its launch can be repeated (new
+ * values in the same buffers) and, with use_graph, is a hipGraph replay.
+ * Per-size-class parameters (class c = problems with ceil(log2(size)) == c,
+ * the last class also takes every larger size) normally come from the
+ * autotuner's table.  Per-problem errors (EINVAL for a bad k, ENAN for a NaN)
+ * are reported by ggms_batch_plan_status after the launch completes.
+ * ------------------------------------------------------------------------ */
+#define GGMS_BATCH_CLASSES 24
+#define GGMS_BATCH_MAX_KS 2048
+
+typedef struct {
+  uint32_t num_buckets;  /* first-pass buckets (16 .. 8048) */
+  uint32_t sample_size;  /* sample keys (pieces + 1 .. 2048) */
+  uint32_t pieces;       /* equal-quantile pieces of the sample (2 .. 64) */
+  uint32_t rebucket;     /* buckets of each later pass over an oversized k-bucket (2 .. 8192; 0 = 8192) */
+} ggms_batch_class;
+
+typedef struct {
+  uint32_t workgroups;     /* persistent workgroups; 0 = one per CU */
+  uint32_t block_threads;  /* threads per workgroup: 256, 512 or 1024 (0 = 1024) */
+  uint32_t use_graph;      /* capture the launch in a hipGraph when the plan is made */
+  uint32_t reserved;
+  uint64_t seed;           /* sampling seed */
+  ggms_batch_class classes[GGMS_BATCH_CLASSES];  /* num_buckets == 0: built-in default */
+} ggms_batch_opts;
+
+typedef struct ggms_batch_plan ggms_batch_plan;
+
+void ggms_batch_opts_default(ggms_batch_opts* opts);
+void ggms_batch_default_classes(ggms_batch_class* classes);
+ggms_status ggms_batch_plan_create(int device, ggms_dtype t, const void* d_values, uint32_t num_problems,
+                                   const uint64_t* h_offsets, const uint32_t* d_ks, const uint32_t* h_koffsets,
+                                   void* d_out, const ggms_batch_opts* opts, ggms_batch_plan** plan);
+/* Asynchronous on stream (NULL: default stream). */
+ggms_status ggms_batch_plan_launch(ggms_batch_plan* plan, void* stream);
+/* Copies per-problem status (num_problems entries) to h_status; returns the first non-OK one. */
+ggms_status ggms_batch_plan_status(ggms_batch_plan* plan, uint32_t* h_status);
+void ggms_batch_plan_destroy(ggms_batch_plan* plan);
+/* One-shot: plan, launch, wait, status, destroy. */
+ggms_status ggms_select_batched(int device, ggms_dtype t, const void* d_values, uint32_t num_problems,
+                                const uint64_t* h_offsets, const uint32_t* d_ks, const uint32_t* h_koffsets,
+                                void* d_out, const ggms_batch_opts* opts);
+
 
 /* ------------------------------------------------------------------------
  * Seeded synthetic inputs (replace DistributedSMOS/generateProblems.cu:43-513,

@@ -1,0 +1,192 @@
+"""GPU parity of batched selection (one persistent launch for many problems)
+against the CPU sort-and-choose oracle, problem by problem, bit-exact.
+
+Each problem has the meaning of one reference call
+(IterativeSMOS::iterativeSMOSWrapper, DistributedSMOS/iterativeSMOS.cu:1579-1612),
+so every output is checked like the reference's compare harness
+(DistributedSMOS/compareIterativeSMOS.cu:186-210) checks one call.
+"""
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+FIN32 = 16384  # keys a workgroup finishes in LDS without bucketing (u32 keys)
+FIN64 = 8192
+ADVERSARIAL = ["uniform", "normal", "lognormal", ("constant", 3.0, 0.0), ("near_constant", 1.0, 0.01),
+               ("mod_index", 5.0, 0.0), "ascending", "cauchy", "mixture", "half_normal"]
+
+
+def _bits(a):
+    a = np.ascontiguousarray(a)
+    return a.view(np.uint64 if a.dtype.itemsize == 8 else np.uint32)
+
+
+def _check(x, offsets, ks, koffsets, got, label="", problems=None):
+    host = x.cpu().numpy()
+    idx = range(len(offsets) - 1) if problems is None else problems
+    for p in idx:
+        a, b = int(offsets[p]), int(offsets[p + 1])
+        ka, kb = int(koffsets[p]), int(koffsets[p + 1])
+        if ka == kb:
+            continue
+        want = oracle.sort_and_choose(host[a:b], ks[ka:kb])
+        bad = np.nonzero(_bits(got[ka:kb]) != _bits(want))[0]
+        assert bad.size == 0, (f"{label} problem {p} (n={b - a}, nk={kb - ka}): {bad.size} wrong; first k="
+                               f"{ks[ka + bad[0]]} got {got[ka + bad[0]]!r} want {want[bad[0]]!r}")
+
+
+def _plan(x, offsets, ks, koffsets, **kw):
+    from ggms.batch import BatchPlan
+    return BatchPlan(x, offsets, ks, koffsets, **kw)
+
+
+def test_batch_mixed_sizes_f32(gpu_ctx):
+    from ggms.batch import make_batch
+    x, off, ks, koff, sizes = make_batch(300, 0, 20, 512, seed=1, dists=ADVERSARIAL)
+    print(f"batch f32: {len(sizes)} problems, {int(off[-1])} values, {len(ks)} ks", flush=True)
+    pl = _plan(x, off, ks, koff)
+    pl.launch()
+    _check(x, off, ks, koff, pl.result(), "f32")
+
+
+@pytest.mark.parametrize("dtype", ["float64", "uint32", "int32"])
+def test_batch_dtypes(gpu_ctx, dtype):
+    from ggms.batch import make_batch
+    dists = ["uniform", ("mod_index", 7.0, 0.0), "ascending"] if dtype != "float64" else ADVERSARIAL
+    x, off, ks, koff, sizes = make_batch(120, 0, 19, 300, seed=2, dtype=dtype, dists=dists)
+    pl = _plan(x, off, ks, koff)
+    pl.launch()
+    _check(x, off, ks, koff, pl.result(), dtype)
+
+
+def _concat(parts, dtype):
+    import torch
+    vals = np.concatenate([v for v, _ in parts]).astype(dtype)
+    off = np.zeros(len(parts) + 1, dtype=np.uint64)
+    off[1:] = np.cumsum([len(v) for v, _ in parts])
+    koff = np.zeros(len(parts) + 1, dtype=np.uint32)
+    koff[1:] = np.cumsum([len(k) for _, k in parts])
+    ks = np.concatenate([np.asarray(k, dtype=np.uint32) for _, k in parts])
+    return torch.from_numpy(vals).cuda(), off, ks, koff
+
+
+@pytest.mark.parametrize("dtype,fin", [(np.float32, FIN32), (np.float64, FIN64)])
+def test_batch_edge_sizes_and_ks(gpu_ctx, dtype, fin):
+    rng = np.random.default_rng(5)
+    parts = []
+    for n in (1, 2, 63, fin - 1, fin, fin + 1, 2 * fin + 3, 5 * fin):
+        v = rng.standard_normal(n).astype(dtype)
+        for nk in (1, 8, 9, 2048):
+            ks = rng.integers(1, n + 1, nk)
+            ks[0] = 1
+            ks[-1] = n
+            parts.append((v, ks))
+    x, off, ks, koff = _concat(parts, dtype)
+    pl = _plan(x, off, ks, koff)
+    pl.launch()
+    _check(x, off, ks, koff, pl.result(), "edges")
+
+
+def test_batch_signed_zeros_infs_and_duplicates(gpu_ctx):
+    rng = np.random.default_rng(6)
+    n = 100_000
+    base = np.array([-0.0, 0.0, np.inf, -np.inf, 1.0, -1.0], dtype=np.float32)
+    v1 = base[rng.integers(0, len(base), n)]
+    v2 = np.where(rng.random(n) < 0.9, np.float32(2.5), rng.standard_normal(n).astype(np.float32))
+    v3 = np.full(n, np.inf, dtype=np.float32)
+    parts = [(v, rng.integers(1, n + 1, 500)) for v in (v1, v2, v3)]
+    x, off, ks, koff = _concat(parts, np.float32)
+    pl = _plan(x, off, ks, koff)
+    pl.launch()
+    _check(x, off, ks, koff, pl.result(), "zeros/infs")
+
+
+def test_batch_small_tables_force_deep_rebucketing(gpu_ctx):
+    """Tiny first-pass tables push big k-buckets through several re-bucketing levels."""
+    from ggms.batch import make_batch
+    x, off, ks, koff, sizes = make_batch(40, 17, 21, 64, seed=7, dists=ADVERSARIAL)
+    classes = {c: dict(num_buckets=16, sample_size=3, pieces=2, rebucket=16) for c in range(24)}
+    pl = _plan(x, off, ks, koff, classes=classes)
+    pl.launch()
+    _check(x, off, ks, koff, pl.result(), "deep")
+
+
+@pytest.mark.parametrize("threads", [256, 512])
+def test_batch_block_sizes(gpu_ctx, threads):
+    from ggms.batch import make_batch
+    x, off, ks, koff, sizes = make_batch(60, 8, 20, 200, seed=8, dists=ADVERSARIAL)
+    pl = _plan(x, off, ks, koff, block_threads=threads, workgroups=37)
+    pl.launch()
+    _check(x, off, ks, koff, pl.result(), f"threads={threads}")
+
+
+def test_batch_graph_replays_with_new_values(gpu_ctx):
+    """A captured plan replayed on new values in the same buffers; every replay checked."""
+    import ctypes as C
+    import torch
+    import ggms
+    from ggms.batch import make_batch
+    x, off, ks, koff, sizes = make_batch(200, 4, 20, 256, seed=9)
+    pl = _plan(x, off, ks, koff, use_graph=True)
+    lib = ggms._lib.load()
+    for rep in range(3):
+        if rep:
+            for p in range(len(sizes)):
+                ptr = x.data_ptr() + int(off[p]) * 4
+                assert lib.ggms_generate(0, ggms.DISTS["mixture" if p % 2 else "normal"], C.c_void_p(ptr),
+                                         int(sizes[p]), 1000 * rep + p, 0.0, 0.0, None) == 0
+            torch.cuda.synchronize()
+        pl.launch()
+        got = pl.result()
+        _check(x, off, ks, koff, got, f"replay {rep}")
+        torch.cuda.synchronize()
+
+
+def test_batch_errors_are_per_problem(gpu_ctx):
+    import ggms
+    rng = np.random.default_rng(10)
+    good = rng.standard_normal(50_000).astype(np.float32)
+    nan_big = good.copy()
+    nan_big[1234] = np.nan
+    nan_small = good[:500].copy()
+    nan_small[7] = np.nan
+    parts = [(good, [1, 2, 50_000]), (nan_big, [5]), (good[:1000], [0]), (good[:1000], [1001]),
+             (nan_small, [3]), (good[:20_000], [17, 20_000])]
+    x, off, ks, koff = _concat(parts, np.float32)
+    pl = _plan(x, off, ks, koff)
+    pl.launch()
+    st = pl.status()
+    assert list(st) == [ggms.OK, ggms.ENAN, ggms.EINVAL, ggms.EINVAL, ggms.ENAN, ggms.OK]
+    got = pl.result(check=False)
+    _check(x, off, ks, koff, got, "errors", problems=[0, 5])
+
+
+def test_select_batched_one_shot_and_invalid_shapes(gpu_ctx):
+    import ctypes as C
+    import ggms
+    from ggms.batch import make_batch, select_batched
+    x, off, ks, koff, sizes = make_batch(50, 0, 18, 100, seed=11)
+    got = select_batched(x, off, ks, koff)
+    _check(x, off, ks, koff, got, "one-shot")
+    import torch
+    lib = ggms._lib.load()
+    d_ks = torch.from_numpy(ks).cuda()
+    d_out = torch.empty(len(ks), dtype=torch.float32, device="cuda")
+    bad_koff = koff.copy()
+    bad_koff[1:] = bad_koff[1:] + 4000  # first problem with more than the ks limit
+    h = C.c_void_p()
+    o = ggms.batch.default_opts()
+    st = lib.ggms_batch_plan_create(0, 0, C.c_void_p(x.data_ptr()), len(sizes),
+                                    off.ctypes.data_as(C.POINTER(C.c_uint64)), C.c_void_p(d_ks.data_ptr()),
+                                    bad_koff.ctypes.data_as(ggms._lib._U32P), C.c_void_p(d_out.data_ptr()),
+                                    C.byref(o), C.byref(h))
+    assert st == ggms.EINVAL
+    o.block_threads = 300
+    st = lib.ggms_batch_plan_create(0, 0, C.c_void_p(x.data_ptr()), len(sizes),
+                                    off.ctypes.data_as(C.POINTER(C.c_uint64)), C.c_void_p(d_ks.data_ptr()),
+                                    koff.ctypes.data_as(ggms._lib._U32P), C.c_void_p(d_out.data_ptr()),
+                                    C.byref(o), C.byref(h))
+    assert st == ggms.EINVAL
