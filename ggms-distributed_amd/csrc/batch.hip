@@ -563,6 +563,22 @@ ggms_status ggms_batch_plan_create(int device, ggms_dtype t, const void* d_value
   int cur = 0;
   if (hipGetDevice(&cur) != hipSuccess) return GGMS_EDEVICE;
   if (hipSetDevice(device) != hipSuccess) return GGMS_EDEVICE;
+  // k ranks are checked here, once per plan: every k of problem p in [1, size_p]
+  if (h_koffsets[num_problems] > 0) {
+    std::vector<uint32_t> hk(h_koffsets[num_problems]);
+    if (hipMemcpy(hk.data(), d_ks, hk.size() * 4, hipMemcpyDeviceToHost) != hipSuccess) {
+      (void)hipSetDevice(cur);
+      return GGMS_EDEVICE;
+    }
+    for (uint32_t p = 0; p < num_problems; ++p) {
+      const uint64_t n = h_offsets[p + 1] - h_offsets[p];
+      for (uint32_t j = h_koffsets[p]; j < h_koffsets[p + 1]; ++j)
+        if (hk[j] < 1 || hk[j] > n) {
+          (void)hipSetDevice(cur);
+          return GGMS_EINVAL;
+        }
+    }
+  }
   ggms_batch_plan* bp = new (std::nothrow) ggms_batch_plan();
   if (!bp) return GGMS_ENOMEM;
   bp->device = device;

@@ -183,8 +183,9 @@ ggms_status ggms_debug_first_pass(ggms_ctx* ctx, ggms_dtype t, const void* d_vec
  * values in the same buffers) and, with use_graph, is a hipGraph replay.
  * Per-size-class parameters (class c = problems with ceil(log2(size)) == c,
  * the last class also takes every larger size) normally come from the
- * autotuner's table.  Per-problem errors (EINVAL for a bad k, ENAN for a NaN)
- * are reported by ggms_batch_plan_status after the launch completes.
+ * autotuner's table.  The k ranks are read and checked when the plan is made
+ * (GGMS_EINVAL for a k outside [1, size]); a NaN in a problem's values is a
+ * per-problem GGMS_ENAN, reported by ggms_batch_plan_status after the launch.
  * ------------------------------------------------------------------------ */
 #define GGMS_BATCH_CLASSES 24
 #define GGMS_BATCH_MAX_KS 2048

@@ -224,7 +224,7 @@ def test_seed_independence(gpu_ctx):
 def test_many_passes(gpu_ctx, dt, dist):
     """Tiny bucket budgets force long re-bucketing chains (every pass-to-pass hand-off is exercised)."""
     import ggms
-    n = 2_000_003
+    n = 8_000_003  # > 16 x 16 x 32768: at least three passes with 16 buckets
     x = ggms.generate(dt, dist, n, 31)
     ks = ggms.generate_ks("uniform_random", 3000, n, 32)
     want = oracle.sort_and_choose(_host(x), ks, threads=8)
