@@ -27,6 +27,22 @@ sys.path.insert(0, os.path.join(ROOT, "oracle"))
 HBM_PEAK_GBS = 8000.0
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` (read + written, scaled by the copy calibration) from the
+    committed byte-counter summary of this workload; (None, None) when it is absent."""
+    path = os.path.join(ROOT, "profiles", "iterative_2p28_f32_normal_1001ks_hbm_bytes.md")
+    if not os.path.exists(path):
+        return None, None
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) >= 5 and cells[0] == f"`{kernel}`":
+            try:
+                return int((float(cells[3]) + float(cells[4])) * 2 ** 20), os.path.relpath(path, ROOT)
+            except ValueError:
+                return None, None
+    return None, None
+
+
 def bench_batched(args):
     """Batched workload: 4096 problems, sizes log-uniform in [2^10, 2^22], each
     uniform / normal / lognormal(0,1) fp32 (seeded per problem), 1..512 random
@@ -83,6 +99,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-check", action="store_true", help="skip the full-size exactness check")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-selection leg")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run one plain device copy of the input (known bytes: 4N read, 4N written) so "
+                         "hardware byte counters can be scaled")
+    ap.add_argument("--no-forms", action="store_true", help="skip the learned and sketch forms")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: the multi-GPU form (one histogram all-reduce "
@@ -99,6 +119,11 @@ def main():
     ks = ggms.generate_ks("uniform_random", args.nk, n, args.seed + 1)
     torch.cuda.synchronize()
     ctx = ggms.Context(0)
+    if args.copy_calibration:
+        y = torch.empty_like(x)
+        y.copy_(x)
+        torch.cuda.synchronize()
+        del y
 
     def timed(opts, steps, warmup):
         for _ in range(warmup):
@@ -138,9 +163,28 @@ def main():
     # dominant kernel: first-pass count (algorithmic bytes = N * 4 per launch)
     c0_ms = st["count_ms"][0]
     achieved = n * bpe / (c0_ms * 1e6)
+    traffic, traffic_src = measured_traffic("k_count0<float>") if args.log2n == 28 else (None, None)
     roofline = {"bound": "hbm", "kernel": "k_count0<float>", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "traffic_source": traffic_src,
                 "algorithmic_bytes_per_launch": n * bpe, "avg_launch_ms": round(c0_ms, 4)}
+    # plain streaming references on the same buffer: a torch sum (reads 4N bytes) and a device copy
+    # (reads and writes 4N bytes), timed with events
+    probe = {}
+    for name, fn, nbytes in (("torch_sum_read", lambda: torch.sum(x), n * bpe),
+                             ("device_copy", lambda: y_probe.copy_(x), 2 * n * bpe)):
+        y_probe = torch.empty_like(x) if name == "device_copy" else None
+        fn()
+        torch.cuda.synchronize()
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ev0.record()
+        for _ in range(5):
+            fn()
+        ev1.record()
+        torch.cuda.synchronize()
+        probe[name + "_GBs"] = round(nbytes * 5 / (ev0.elapsed_time(ev1) * 1e6), 1)
+        del y_probe
+    roofline["streaming_probes"] = probe
 
     result = {
         "metric": "elements/s + HBM GB/s, 2^28 fp32 1001 ks: learned vs linear splitters",
@@ -165,7 +209,7 @@ def main():
     }
 
     # learned-splitter form (same engine, MLP pivots) and its no-network ablation
-    for form in ("learned", "sketch"):
+    for form in () if args.no_forms else ("learned", "sketch"):
         try:
             f_opts = ggms.default_opts(splitter=form)
             out_f, t_f, tall_f = timed(f_opts, args.steps, args.warmup)

@@ -24,9 +24,9 @@ template <class T> struct KeyTraits;
 template <> struct KeyTraits<float> {
   using K = uint32_t;
   static constexpr bool kFloat = true;
-  static GGMS_HD K enc(float v) {
+  static GGMS_HD K enc(float v) {  // negative: ~u; otherwise u | sign bit
     uint32_t u = __builtin_bit_cast(uint32_t, v);
-    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+    return u ^ ((uint32_t)((int32_t)u >> 31) | 0x80000000u);
   }
   static GGMS_HD float dec(K k) {
     uint32_t u = (k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k;
@@ -41,7 +41,7 @@ template <> struct KeyTraits<double> {
   static constexpr bool kFloat = true;
   static GGMS_HD K enc(double v) {
     uint64_t u = __builtin_bit_cast(uint64_t, v);
-    return (u & 0x8000000000000000ull) ? ~u : (u | 0x8000000000000000ull);
+    return u ^ ((uint64_t)((int64_t)u >> 63) | 0x8000000000000000ull);
   }
   static GGMS_HD double dec(K k) {
     uint64_t u = (k & 0x8000000000000000ull) ? (k & 0x7FFFFFFFFFFFFFFFull) : ~k;

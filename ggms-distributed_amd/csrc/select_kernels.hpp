@@ -214,6 +214,71 @@ __device__ __forceinline__ void stream_tile(const V* __restrict__ x, uint32_t t0
   for (uint32_t i = tail0 + threadIdx.x; i < t1; i += nt) f(x[i], i);
 }
 
+// Vector form: fv(const V (&e)[N]) receives the N values of each 16-byte load
+// (so per-element table lookups of one load can be issued together), f1(V)
+// the ragged tail.  Same alignment rules as stream_tile.
+template <class V, class FV, class F1>
+__device__ __forceinline__ void stream_tile_vec(const V* __restrict__ x, uint32_t t0, uint32_t t1, FV&& fv, F1&& f1) {
+  constexpr int N = Vec16<V>::N;
+  const uint32_t nt = blockDim.x;
+  const uint4* xv = reinterpret_cast<const uint4*>(x);
+  uint32_t v1 = t1 / N;
+  uint32_t v = t0 / N + threadIdx.x;
+  for (; v + 3 * nt < v1; v += 4 * nt) {
+    uint4 q0 = xv[v];
+    uint4 q1 = xv[v + nt];
+    uint4 q2 = xv[v + 2 * nt];
+    uint4 q3 = xv[v + 3 * nt];
+    V e[N];
+    Vec16<V>::unpack(q0, e);
+    fv(e);
+    Vec16<V>::unpack(q1, e);
+    fv(e);
+    Vec16<V>::unpack(q2, e);
+    fv(e);
+    Vec16<V>::unpack(q3, e);
+    fv(e);
+  }
+  for (; v < v1; v += nt) {
+    uint4 q0 = xv[v];
+    V e[N];
+    Vec16<V>::unpack(q0, e);
+    fv(e);
+  }
+  uint32_t tail0 = (v1 * N > t0) ? v1 * N : t0;
+  for (uint32_t i = tail0 + threadIdx.x; i < t1; i += nt) f1(x[i]);
+}
+
+template <class V, class FV, class F1>
+__device__ __forceinline__ void stream_tile_vec_any(const V* __restrict__ x, uint32_t t0, uint32_t t1, FV&& fv,
+                                                    F1&& f1) {
+  if ((reinterpret_cast<uintptr_t>(x) & 15u) == 0) {
+    stream_tile_vec<V>(x, t0, t1, fv, f1);
+  } else {
+    for (uint32_t i = t0 + threadIdx.x; i < t1; i += blockDim.x) f1(x[i]);
+  }
+}
+
+// hist[b[j]] += 1 for N buckets per lane (every lane of the wave active for
+// all N).  One compare + ballot per bucket decides whether any of the N
+// needs duplicate peeling; the common case is N plain LDS atomics.
+template <int N>
+__device__ __forceinline__ void lds_count_n(uint32_t* hist, const uint32_t (&b)[N]) {
+  bool peel = false;
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+    uint32_t b0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)b[j]);
+    peel |= __builtin_popcountll(wave_ballot(b[j] == b0)) >= kPeelMin;
+  }
+  if (peel) {
+#pragma unroll
+    for (int j = 0; j < N; ++j) lds_count(hist, b[j]);
+  } else {
+#pragma unroll
+    for (int j = 0; j < N; ++j) atomicAdd(&hist[b[j]], 1u);
+  }
+}
+
 // Caller vectors may start at any element; 16-byte loads need 16-byte
 // alignment, so an unaligned base takes the one-element-per-lane path.
 template <class V, class F>
@@ -280,11 +345,14 @@ template <class K> __device__ __forceinline__ MapParam<K> to_param(const Range<K
   return q;
 }
 
+// Branch-free: mult == 0 (identity map) gives mulhi == 0 and adds t instead,
+// so a wave never splits on the map kind.
 template <class K> __device__ __forceinline__ uint32_t param_bucket(const MapParam<K>& q, K key) {
   K d = key - q.lo;
   uint32_t t = sizeof(K) == 4 ? (uint32_t)d : (uint32_t)(d >> q.sh);
-  uint32_t off = q.mult ? __umulhi(t, q.mult) : t;
-  return q.base + off;
+  uint32_t h = __umulhi(t, q.mult);
+  uint32_t id = q.mult == 0 ? t : 0u;
+  return q.base + h + id;
 }
 
 template <class K> struct Pass0Map {
@@ -324,6 +392,43 @@ template <class K> struct Pass0Map {
     return r;
   }
   __device__ __forceinline__ uint32_t bucket(K key) const { return param_bucket<K>(prm[range_of(key)], key); }
+
+  // Buckets of N keys: the N level-1 reads are issued together, one ballot
+  // covers the rare non-leaf cells of all N, then the N parameter reads.
+  template <int N> __device__ __forceinline__ void buckets(const K (&key)[N], uint32_t (&b)[N]) const {
+    K e[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) e[j] = lut[(uint32_t)(key[j] >> G::S1)];
+    uint32_t r[N];
+    bool nl = false;
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      r[j] = (uint32_t)((e[j] >> G::S1) & 0x7FF) + ((key[j] & G::M1) > (e[j] & G::M1) ? 1u : 0u);
+      nl |= (e[j] & G::FLAG) != 0;
+    }
+    if (wave_ballot(nl)) {
+#pragma unroll
+      for (int j = 0; j < N; ++j) {
+        K ej = e[j];
+        if (ej & G::FLAG) {
+          if (ej & G::TABLE) {
+            ej = lut[G::L1CELLS + (((uint32_t)(ej & (G::TABLE - 1))) << 6) + ((uint32_t)(key[j] >> G::S2) & 63u)];
+            r[j] = (uint32_t)((ej >> G::S2) & 0x1FFFF) + ((key[j] & G::M2) > (ej & G::M2) ? 1u : 0u);
+          }
+          if (ej & G::FLAG) {
+            uint32_t lo = (uint32_t)(ej & 0x7FF), hi = (uint32_t)((ej >> 11) & 0x7FF);
+            while (lo < hi) {
+              uint32_t mid = (lo + hi + 1) >> 1;
+              if (prm[mid].lo <= key[j]) lo = mid; else hi = mid - 1;
+            }
+            r[j] = lo;
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < N; ++j) b[j] = param_bucket<K>(prm[r[j]], key[j]);
+  }
 };
 
 // Range of the last start <= key among starts[lo..hi].
@@ -434,7 +539,18 @@ __global__ void __launch_bounds__(1024) k_count0(const T* __restrict__ x, const 
   __syncthreads();
   uint32_t t0 = blockIdx.x * tile;
   uint32_t t1 = t0 + tile < n ? t0 + tile : n;
-  stream_tile_any<T>(x, t0, t1, [&](T v, uint32_t) { lds_count(hist, mp.bucket(KeyTraits<T>::enc(v))); });
+  constexpr int N = Vec16<T>::N;
+  stream_tile_vec_any<T>(
+      x, t0, t1,
+      [&](const T (&v)[N]) {
+        K k[N];
+#pragma unroll
+        for (int j = 0; j < N; ++j) k[j] = KeyTraits<T>::enc(v[j]);
+        uint32_t b[N];
+        mp.template buckets<N>(k, b);
+        lds_count_n<N>(hist, b);
+      },
+      [&](T v) { lds_count(hist, mp.bucket(KeyTraits<T>::enc(v))); });
   __syncthreads();
   uint32_t* row = rows + (size_t)blockIdx.x * B;
   for (uint32_t i = threadIdx.x; i < B; i += blockDim.x) row[i] = hist[i];
@@ -895,18 +1011,48 @@ __global__ void __launch_bounds__(1024) k_compact0(const T* __restrict__ x, cons
   __syncthreads();
   uint32_t t0 = blockIdx.x * tile;
   uint32_t t1 = t0 + tile < n ? t0 + tile : n;
-  stream_tile_any<T>(x, t0, t1, [&](T v, uint32_t) {
-    K key = KeyTraits<T>::enc(v);
-    bool act = slot[mp.bucket(key)] != kNone16;
-    uint64_t m = wave_ballot(act);
-    if (m) {
-      uint32_t first = (uint32_t)__builtin_ctzll(m);
-      uint32_t base = 0;
-      if (lane_id() == first) base = atomicAdd(&cursor, (uint32_t)__builtin_popcountll(m));
-      base = readlane_u32(base, first);
-      if (act) out[base + mask_below(m)] = key;
-    }
-  });
+  constexpr int N = Vec16<T>::N;
+  stream_tile_vec_any<T>(
+      x, t0, t1,
+      [&](const T (&v)[N]) {
+        K k[N];
+#pragma unroll
+        for (int j = 0; j < N; ++j) k[j] = KeyTraits<T>::enc(v[j]);
+        uint32_t b[N];
+        mp.template buckets<N>(k, b);
+        bool act[N];
+        uint64_t m[N];
+        uint32_t tot = 0;
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+          act[j] = slot[b[j]] != kNone16;
+          m[j] = wave_ballot(act[j]);
+          tot += (uint32_t)__builtin_popcountll(m[j]);
+        }
+        if (tot) {  // one run claimed for the wave's survivors of all N loads
+          uint32_t first = (uint32_t)__builtin_ctzll(wave_ballot(true));
+          uint32_t base = 0;
+          if (lane_id() == first) base = atomicAdd(&cursor, tot);
+          base = readlane_u32(base, first);
+#pragma unroll
+          for (int j = 0; j < N; ++j) {
+            if (act[j]) out[base + mask_below(m[j])] = k[j];
+            base += (uint32_t)__builtin_popcountll(m[j]);
+          }
+        }
+      },
+      [&](T v) {
+        K key = KeyTraits<T>::enc(v);
+        bool act = slot[mp.bucket(key)] != kNone16;
+        uint64_t m = wave_ballot(act);
+        if (m) {
+          uint32_t first = (uint32_t)__builtin_ctzll(m);
+          uint32_t base = 0;
+          if (lane_id() == first) base = atomicAdd(&cursor, (uint32_t)__builtin_popcountll(m));
+          base = readlane_u32(base, first);
+          if (act) out[base + mask_below(m)] = key;
+        }
+      });
 }
 
 // Pass 1 reads the append-layout buffer: an element's range is the slot of its
