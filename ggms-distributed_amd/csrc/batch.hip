@@ -308,15 +308,30 @@ __global__ void __launch_bounds__(1024) k_batch(BatchArgs a) {
     K* samp = L.A;
     const K lowest = KT::kFloat ? KT::kNegInf : (K)0;
     const K highest = KT::kFloat ? KT::kPosInf : KeyMax<K>::v;
-    for (uint32_t i = threadIdx.x; i < Sp; i += blockDim.x) {
-      K key = KeyMax<K>::v;
-      if (i < S) {
-        uint32_t pos = (uint32_t)(((uint64_t)philox_draw(a.seed ^ ((uint64_t)p << 20), kStreamSample, i).v[0] * n) >> 32);
-        key = KT::enc(x[pos]);
-        key = key < lowest ? lowest : key;
-        key = key > highest ? highest : key;
+    for (uint32_t i0 = threadIdx.x; i0 < Sp; i0 += 2 * blockDim.x) {  // S <= 2048: two loads in flight
+      uint32_t pos[2];
+      T v[2];
+#pragma unroll
+      for (uint32_t u = 0; u < 2; ++u) {
+        uint32_t i = i0 + u * blockDim.x;
+        pos[u] = i < S ? (uint32_t)(((uint64_t)philox_draw(a.seed ^ ((uint64_t)p << 20), kStreamSample, i).v[0] * n) >> 32)
+                       : 0u;
       }
-      samp[i] = key;
+#pragma unroll
+      for (uint32_t u = 0; u < 2; ++u) v[u] = x[pos[u]];
+#pragma unroll
+      for (uint32_t u = 0; u < 2; ++u) {
+        uint32_t i = i0 + u * blockDim.x;
+        if (i < Sp) {
+          K key = KeyMax<K>::v;
+          if (i < S) {
+            key = KT::enc(v[u]);
+            key = key < lowest ? lowest : key;
+            key = key > highest ? highest : key;
+          }
+          samp[i] = key;
+        }
+      }
     }
     __syncthreads();
     bitonic_sort_lds<K>(samp, Sp);
@@ -395,7 +410,18 @@ __global__ void __launch_bounds__(1024) k_batch(BatchArgs a) {
     mp.prm = L.prm;
     for (uint32_t i = threadIdx.x; i < B0; i += blockDim.x) L.H[i] = 0;
     __syncthreads();
-    stream_tile_any<T>(x, 0, n, [&](T v, uint32_t) { lds_count(L.H, mp.bucket(KT::enc(v))); });
+    constexpr int NV = Vec16<T>::N;
+    stream_tile_vec_any<T>(
+        x, 0, n,
+        [&](const T (&v)[NV]) {
+          K k[NV];
+#pragma unroll
+          for (int j = 0; j < NV; ++j) k[j] = KT::enc(v[j]);
+          uint32_t b[NV];
+          mp.template buckets<NV>(k, b);
+          lds_count_n<NV>(L.H, b);
+        },
+        [&](T v) { lds_count(L.H, mp.bucket(KT::enc(v))); });
     __syncthreads();
     if (KT::kFloat && (L.H[0] | L.H[B0 - 1])) {
       if (threadIdx.x == 0) a.status[p] = GGMS_ENAN;
@@ -420,17 +446,85 @@ __global__ void __launch_bounds__(1024) k_batch(BatchArgs a) {
     batch_plan<T>(L, B0, 0, nk, 0, 0, interval0, out, stack, &s_nkids, slot, L.cursor, L.runb, scan_s);
     uint32_t nkids = s_nkids;
     if (nkids) {
-      stream_tile_any<T>(x, 0, n, [&](T v, uint32_t) {
-        K key = KT::enc(v);
-        uint32_t s = slot[mp.bucket(key)];
-        bool act = s != kNone16;
-        uint32_t pos = lds_claim(L.cursor, s, act);
-        if (act) sA[pos] = key;
-      });
+      stream_tile_vec_any<T>(
+          x, 0, n,
+          [&](const T (&v)[NV]) {
+            K k[NV];
+#pragma unroll
+            for (int j = 0; j < NV; ++j) k[j] = KT::enc(v[j]);
+            uint32_t b[NV];
+            mp.template buckets<NV>(k, b);
+#pragma unroll
+            for (int j = 0; j < NV; ++j) {
+              uint32_t s = slot[b[j]];
+              bool act = s != kNone16;
+              uint32_t pos = lds_claim(L.cursor, s, act);
+              if (act) sA[pos] = k[j];
+            }
+          },
+          [&](T v) {
+            K key = KT::enc(v);
+            uint32_t s = slot[mp.bucket(key)];
+            bool act = s != kNone16;
+            uint32_t pos = lds_claim(L.cursor, s, act);
+            if (act) sA[pos] = key;
+          });
       __threadfence_block();
       __syncthreads();
     }
-    if (threadIdx.x == 0) s_sp = nkids;
+    // ---- first-pass children.  They sit in sA in bucket order, so their key
+    // intervals are disjoint and increasing: a run of consecutive small
+    // children is loaded and sorted as one batch and each k is read off at
+    // (child offset in the batch + rank in child).  Big children move to the
+    // front of the stack for re-bucketing.
+    {
+      uint32_t* c_len = L.H;
+      uint32_t* c_j0 = L.H + kBatchMaxKs;
+      uint32_t* c_j1 = L.H + 2 * kBatchMaxKs;
+      uint32_t* c_reg = L.H + 3 * kBatchMaxKs;
+      for (uint32_t c = threadIdx.x; c < nkids; c += blockDim.x) {
+        const BatchItem<K> it = stack[c];
+        c_len[c] = it.len;
+        c_j0[c] = it.j0;
+        c_j1[c] = it.j1;
+        c_reg[c] = it.region;
+      }
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        uint32_t nbig = 0;
+        for (uint32_t c = 0; c < nkids; ++c)
+          if (c_len[c] > FIN) stack[nbig++] = stack[c];
+        s_sp = nbig;
+      }
+      __threadfence_block();
+      uint32_t c = 0;
+      while (c < nkids) {  // every thread walks the same LDS table: uniform
+        if (c_len[c] > FIN) {
+          ++c;
+          continue;
+        }
+        uint32_t e = c, tot = 0;
+        while (e < nkids && c_len[e] <= FIN && tot + c_len[e] <= FIN) tot += c_len[e++];
+        const K* src = sA + c_reg[c];
+        for (uint32_t i = threadIdx.x; i < tot; i += blockDim.x) L.A[i] = src[i];
+        __syncthreads();
+        if (e - c == 1) {
+          batch_finish<T>(L, tot, (K)0, KeyMax<K>::v, c_j0[c], c_j1[c], out, hist256, bcast);
+        } else {
+          const uint32_t M = pow2ceil_u32(tot);
+          for (uint32_t i = tot + threadIdx.x; i < M; i += blockDim.x) L.A[i] = KeyMax<K>::v;
+          __syncthreads();
+          bitonic_sort_lds<K>(L.A, M);
+          for (uint32_t q = c; q < e; ++q) {
+            const uint32_t off = c_reg[q] - c_reg[c];
+            for (uint32_t j = c_j0[q] + threadIdx.x; j < c_j1[q]; j += blockDim.x)
+              out[L.Ko[j]] = KeyTraits<T>::dec(L.A[off + L.Kr[j] - 1]);
+          }
+          __syncthreads();
+        }
+        c = e;
+      }
+    }
     __syncthreads();
     // ---- work stack: finish small items, re-bucket large ones -----------------
     for (;;) {

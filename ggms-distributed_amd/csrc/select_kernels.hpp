@@ -118,20 +118,80 @@ __device__ void block_incl_scan_lds(uint32_t* a, uint32_t L, uint32_t* scratch) 
   __syncthreads();
 }
 
+// Bitonic sort (ascending) of M keys in LDS, M a power of two, by the whole
+// workgroup.  When M = E * blockDim with E in {4, 8, 16}, thread t owns keys
+// [t*E, t*E + E): every compare-exchange distance below E runs in registers
+// (one load, all short-distance stages, one store), so only distances >= E
+// cost a barrier each.
 template <class K>
-__device__ void bitonic_sort_lds(K* s, uint32_t M) {
-  for (uint32_t size = 2; size <= M; size <<= 1) {
-    for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
-      for (uint32_t i = threadIdx.x; i < (M >> 1); i += blockDim.x) {
-        uint32_t lo = 2 * i - (i & (stride - 1));
-        uint32_t hi = lo + stride;
-        bool asc = (lo & size) == 0;
-        K a = s[lo], b = s[hi];
-        if ((a > b) == asc) {
-          s[lo] = b;
-          s[hi] = a;
+__device__ __forceinline__ void bitonic_stage_lds(K* s, uint32_t M, uint32_t size, uint32_t stride) {
+  for (uint32_t i = threadIdx.x; i < (M >> 1); i += blockDim.x) {
+    uint32_t lo = 2 * i - (i & (stride - 1));
+    uint32_t hi = lo + stride;
+    bool asc = (lo & size) == 0;
+    K a = s[lo], b = s[hi];
+    if ((a > b) == asc) {
+      s[lo] = b;
+      s[hi] = a;
+    }
+  }
+}
+
+template <class K, int E>
+__device__ __forceinline__ void bitonic_reg_merge(K (&r)[E], uint32_t base, uint32_t size, uint32_t stride_max) {
+#pragma unroll
+  for (int st = E / 2; st > 0; st >>= 1) {
+    if ((uint32_t)st <= stride_max) {
+#pragma unroll
+      for (int i = 0; i < E; ++i) {
+        if ((i & st) == 0) {
+          bool asc = ((base + (uint32_t)i) & size) == 0;
+          K a = r[i], b = r[i + st];
+          bool sw = (a > b) == asc;
+          r[i] = sw ? b : a;
+          r[i + st] = sw ? a : b;
         }
       }
+    }
+  }
+}
+
+template <class K, int E>
+__device__ void bitonic_sort_lds_reg(K* s, uint32_t M) {
+  const uint32_t base = threadIdx.x * E;
+  K r[E];
+#pragma unroll
+  for (int i = 0; i < E; ++i) r[i] = s[base + i];
+  for (uint32_t size = 2; size <= (uint32_t)E; size <<= 1) bitonic_reg_merge<K, E>(r, base, size, size >> 1);
+#pragma unroll
+  for (int i = 0; i < E; ++i) s[base + i] = r[i];
+  __syncthreads();
+  for (uint32_t size = 2 * E; size <= M; size <<= 1) {
+    for (uint32_t stride = size >> 1; stride >= (uint32_t)E; stride >>= 1) {
+      bitonic_stage_lds<K>(s, M, size, stride);
+      __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < E; ++i) r[i] = s[base + i];
+    bitonic_reg_merge<K, E>(r, base, size, E / 2);
+#pragma unroll
+    for (int i = 0; i < E; ++i) s[base + i] = r[i];
+    __syncthreads();
+  }
+}
+
+template <class K>
+__device__ void bitonic_sort_lds(K* s, uint32_t M) {
+  const uint32_t e = M / blockDim.x;
+  if (M == e * blockDim.x && (e == 4 || e == 8 || (e == 16 && sizeof(K) == 4))) {
+    if (e == 4) bitonic_sort_lds_reg<K, 4>(s, M);
+    else if (e == 8) bitonic_sort_lds_reg<K, 8>(s, M);
+    else bitonic_sort_lds_reg<K, 16>(s, M);
+    return;
+  }
+  for (uint32_t size = 2; size <= M; size <<= 1) {
+    for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+      bitonic_stage_lds<K>(s, M, size, stride);
       __syncthreads();
     }
   }

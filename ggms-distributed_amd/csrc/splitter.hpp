@@ -33,9 +33,14 @@ constexpr int kMlpParams = kMlpHidden * kMlpIn + kMlpHidden + kMlpHidden + 1;
 // LDS of k_plan0: sorted sample, pivots (<= 2048), range starts, requested and
 // final bucket counts (at most 2 * 1025 + 6 ranges).
 constexpr uint32_t kPlan0MaxRanges = 2064;
+// learned pivots are evaluated in chunks of this many (features of a chunk in LDS)
+constexpr uint32_t kMlpChunk = 128;
+constexpr uint32_t kMlpFeatStride = kMlpIn + 1;
 template <class K> __host__ __device__ constexpr size_t plan0_lds_bytes(uint32_t S_pad) {
   return (size_t)S_pad * sizeof(K) + 2048 * sizeof(K) + kPlan0MaxRanges * sizeof(K) +
-         (kPlan0MaxRanges * 8 > kMlpParams * 4 ? kPlan0MaxRanges * 8 : kMlpParams * 4);
+         (kPlan0MaxRanges * 8 > (kMlpParams + kMlpChunk * kMlpFeatStride) * 4
+              ? kPlan0MaxRanges * 8
+              : (kMlpParams + kMlpChunk * kMlpFeatStride) * 4);
 }
 
 template <class K>
@@ -45,44 +50,83 @@ __device__ __forceinline__ double key_to_double<uint32_t>(uint32_t k) { return (
 template <>
 __device__ __forceinline__ double key_to_double<uint64_t>(uint64_t k) { return (double)k; }
 
-// Learned pivot: the MLP reads a window of 64 consecutive sorted sample keys
+// Learned pivots: the MLP reads a window of 64 consecutive sorted sample keys
 // centred on the target quantile, normalised to [0, 1] by the window's own
 // extremes, plus the target's fractional position inside the window, and
 // predicts where between the two order statistics that bracket the target
 // the true quantile lies (sigmoid output).  The prediction therefore always
 // falls between those two sample keys.
+// All learned pivots: q[0] = s[0], q[Pl] = s[S-1] and, for j = 1 .. Pl-1,
+// the network's pivot for rank fraction j / Pl.  Pivots go in chunks: the
+// chunk's features are built in LDS (one thread per feature), then one thread
+// per (pivot, hidden unit) evaluates a hidden unit and the 32 units of a pivot
+// are summed across lanes.  feat: kMlpChunk * kMlpFeatStride floats of LDS.
 template <class K>
-__device__ K learned_pivot(const K* s, uint32_t S, double rank_frac, const float* w) {
-  double pos = rank_frac * (double)(S - 1);
-  int ip = (int)pos;
+__device__ __forceinline__ void learned_window(uint32_t j, uint32_t Pl, uint32_t S, double& pos, int& ip, int& w0) {
+  pos = (double)j / (double)Pl * (double)(S - 1);
+  ip = (int)pos;
   if (ip >= (int)S - 1) ip = (int)S - 2;
   if (ip < 0) ip = 0;
-  double fr = pos - (double)ip;
-  int w0 = ip - kMlpWin / 2 + 1;
+  w0 = ip - kMlpWin / 2 + 1;
   if (w0 < 0) w0 = 0;
   if (w0 + kMlpWin > (int)S) w0 = (int)S - kMlpWin;
-  K klo = s[w0], khi = s[w0 + kMlpWin - 1];
-  double span = key_to_double<K>(khi - klo);
-  float in[kMlpIn];
-  for (int i = 0; i < kMlpWin; ++i)
-    in[i] = span > 0 ? (float)(key_to_double<K>(s[w0 + i] - klo) / span) : 0.0f;
-  in[kMlpWin] = (float)(((double)(ip - w0) + fr) / (double)(kMlpWin - 1));
+}
+
+template <class K>
+__device__ void learned_pivots(const K* s, uint32_t S, uint32_t Pl, const float* w, K* q, float* feat) {
   const float* W1 = w;
   const float* b1 = w + kMlpHidden * kMlpIn;
   const float* W2 = b1 + kMlpHidden;
   const float b2 = W2[kMlpHidden];
-  float o = b2;
-  for (int h = 0; h < kMlpHidden; ++h) {
-    float z = b1[h];
-    for (int i = 0; i < kMlpIn; ++i) z += W1[h * kMlpIn + i] * in[i];
-    o += W2[h] * (z > 0.0f ? z : 0.0f);
+  if (threadIdx.x == 0) {
+    q[0] = s[0];
+    q[Pl] = s[S - 1];
   }
-  float f = 1.0f / (1.0f + __expf(-o));
-  K a = s[ip], b = s[ip + 1];
-  double d = key_to_double<K>(b - a) * (double)f;
-  K off = (K)d;
-  if (off > (K)(b - a)) off = (K)(b - a);
-  return (K)(a + off);
+  for (uint32_t c0 = 1; c0 < Pl; c0 += kMlpChunk) {
+    const uint32_t nc = Pl - c0 < kMlpChunk ? Pl - c0 : kMlpChunk;
+    for (uint32_t t = threadIdx.x; t < nc * kMlpIn; t += blockDim.x) {
+      const uint32_t pj = t / kMlpIn, i = t % kMlpIn;
+      double pos;
+      int ip, w0;
+      learned_window<K>(c0 + pj, Pl, S, pos, ip, w0);
+      float f;
+      if (i < (uint32_t)kMlpWin) {
+        K klo = s[w0], khi = s[w0 + kMlpWin - 1];
+        double span = key_to_double<K>(khi - klo);
+        f = span > 0 ? (float)(key_to_double<K>(s[w0 + i] - klo) / span) : 0.0f;
+      } else {
+        f = (float)(((double)(ip - w0) + (pos - (double)ip)) / (double)(kMlpWin - 1));
+      }
+      feat[pj * kMlpFeatStride + i] = f;
+    }
+    __syncthreads();
+    const uint32_t work = (nc * kMlpHidden + blockDim.x - 1) / blockDim.x * blockDim.x;
+    for (uint32_t t = threadIdx.x; t < work; t += blockDim.x) {
+      const uint32_t pj = t / kMlpHidden, h = t % kMlpHidden;
+      float contrib = 0.0f;
+      if (pj < nc) {
+        const float* in = feat + pj * kMlpFeatStride;
+        float z = b1[h];
+        for (int i = 0; i < kMlpIn; ++i) z += W1[h * kMlpIn + i] * in[i];
+        contrib = W2[h] * (z > 0.0f ? z : 0.0f);
+      }
+#pragma unroll
+      for (int d = kMlpHidden / 2; d > 0; d >>= 1) contrib += __shfl_xor(contrib, d, kMlpHidden);
+      if (pj < nc && h == 0) {
+        const uint32_t j = c0 + pj;
+        float f = 1.0f / (1.0f + __expf(-(b2 + contrib)));
+        double pos;
+        int ip, w0;
+        learned_window<K>(j, Pl, S, pos, ip, w0);
+        K lo = s[ip], hi = s[ip + 1];
+        double d = key_to_double<K>(hi - lo) * (double)f;
+        K off = (K)d;
+        if (off > (K)(hi - lo)) off = (K)(hi - lo);
+        q[j] = (K)(lo + off);
+      }
+    }
+    __syncthreads();
+  }
 }
 
 template <class T>
@@ -102,25 +146,44 @@ __global__ void __launch_bounds__(1024) k_plan0(const T* __restrict__ x, Plan0Ar
   const K lowest = KT::kFloat ? KT::kNegInf : (K)0;
   const K highest = KT::kFloat ? KT::kPosInf : KeyMax<K>::v;
 
-  // 1. sample
-  for (uint32_t i = threadIdx.x; i < Spad; i += blockDim.x) {
-    K key = KeyMax<K>::v;
-    if (i < S) {
-      uint32_t pos;
-      if (a.mode == kModeStrided) {
-        uint32_t S1 = a.strideS;
-        uint32_t stride = n / S1 > 0 ? n / S1 : 1;
-        uint32_t j = (uint32_t)(((uint64_t)philox_draw(a.seed, kStreamSubSample, i).v[0] * S1) >> 32);
-        uint64_t p = (uint64_t)j * stride;
-        pos = p < n ? (uint32_t)p : n - 1;
-      } else {
-        pos = (uint32_t)(((uint64_t)philox_draw(a.seed, kStreamSample, i).v[0] * n) >> 32);
+  // 1. sample: up to 8 positions per thread, all loads issued before the
+  //    first use (random reads over the whole vector: each one is a full
+  //    memory round trip, so they must overlap)
+  constexpr uint32_t kSampleUnroll = 8;
+  for (uint32_t i0 = threadIdx.x; i0 < Spad; i0 += kSampleUnroll * blockDim.x) {
+    uint32_t pos[kSampleUnroll];
+#pragma unroll
+    for (uint32_t u = 0; u < kSampleUnroll; ++u) {
+      uint32_t i = i0 + u * blockDim.x;
+      uint32_t p = 0;
+      if (i < S) {
+        if (a.mode == kModeStrided) {
+          uint32_t S1 = a.strideS;
+          uint32_t stride = n / S1 > 0 ? n / S1 : 1;
+          uint32_t j = (uint32_t)(((uint64_t)philox_draw(a.seed, kStreamSubSample, i).v[0] * S1) >> 32);
+          uint64_t pp = (uint64_t)j * stride;
+          p = pp < n ? (uint32_t)pp : n - 1;
+        } else {
+          p = (uint32_t)(((uint64_t)philox_draw(a.seed, kStreamSample, i).v[0] * n) >> 32);
+        }
       }
-      key = KT::enc(x[pos]);
-      if (key < lowest) key = lowest;
-      if (key > highest) key = highest;
+      pos[u] = p;
     }
-    s[i] = key;
+    T v[kSampleUnroll];
+#pragma unroll
+    for (uint32_t u = 0; u < kSampleUnroll; ++u) v[u] = x[pos[u]];
+#pragma unroll
+    for (uint32_t u = 0; u < kSampleUnroll; ++u) {
+      uint32_t i = i0 + u * blockDim.x;
+      if (i >= Spad) break;
+      K key = KeyMax<K>::v;
+      if (i < S) {
+        key = KT::enc(v[u]);
+        if (key < lowest) key = lowest;
+        if (key > highest) key = highest;
+      }
+      s[i] = key;
+    }
   }
   __syncthreads();
   bitonic_sort_lds<K>(s, Spad);
@@ -160,16 +223,19 @@ __global__ void __launch_bounds__(1024) k_plan0(const T* __restrict__ x, Plan0Ar
       for (uint32_t i = threadIdx.x; i < (uint32_t)kMlpParams; i += blockDim.x) wl[i] = a.mlp[i];
       __syncthreads();
     }
-    for (uint32_t j = threadIdx.x; j <= Pl; j += blockDim.x) {
-      K v;
-      if (j == 0) v = s[0];
-      else if (j == Pl) v = s[S - 1];
-      else if (a.mode == kModeLearned) v = learned_pivot<K>(s, S, (double)j / (double)Pl, wl);
-      else {
-        double pos = (double)j / (double)Pl * (double)(S - 1);
-        v = s[(uint32_t)(pos + 0.5)];
+    if (a.mode == kModeLearned) {
+      learned_pivots<K>(s, S, Pl, wl, q, reinterpret_cast<float*>(wl + kMlpParams));
+    } else {
+      for (uint32_t j = threadIdx.x; j <= Pl; j += blockDim.x) {
+        K v;
+        if (j == 0) v = s[0];
+        else if (j == Pl) v = s[S - 1];
+        else {
+          double pos = (double)j / (double)Pl * (double)(S - 1);
+          v = s[(uint32_t)(pos + 0.5)];
+        }
+        q[j] = v;
       }
-      q[j] = v;
     }
     __syncthreads();
     // predictions may be out of order by a key or two; sort them
@@ -186,51 +252,60 @@ __global__ void __launch_bounds__(1024) k_plan0(const T* __restrict__ x, Plan0Ar
   // the network weights (only used before this point) share this space
   uint32_t* rreq = reinterpret_cast<uint32_t*>(rlo + kPlan0MaxRanges);  // [kPlan0MaxRanges]
   uint32_t* rnb = rreq + kPlan0MaxRanges;                             // [kPlan0MaxRanges]
-  if (threadIdx.x == 0) {
-    enum { kSingle = 0, kEdge = 1, kInterior = 2, kNan = 3 };
-    uint32_t m_int = 0;
-    uint32_t R = 0;
-    uint32_t share = 1;
-    for (int emit = 0; emit < 2; ++emit) {
-      auto cb = [&](K lo, int kind) {
-        if (!emit) {
-          if (kind == kInterior) m_int++;
-          return;
-        }
-        rlo[R] = lo;
-        rreq[R] = kind == kInterior ? share : kind == kEdge ? a.E : 1u;
-        R++;
-      };
-      if (KT::kFloat) cb((K)0, kNan);
-      if (q[0] > lowest) cb(lowest, kEdge);
-      uint32_t i = 0;
-      while (i < Q) {
-        K v = q[i];
-        uint32_t j = i;
-        while (j + 1 < Q && q[j + 1] == v) ++j;
-        if (j + 1 >= Q) {
-          cb(v, kSingle);
-          break;
-        }
-        K nxt = q[j + 1];
-        if (j > i) {
-          cb(v, kSingle);
-          if ((K)(nxt - v) >= 2) cb((K)(v + 1), kInterior);
-        } else {
-          cb(v, kInterior);
-        }
-        i = j + 1;
-      }
-      if (q[Q - 1] < highest) cb((K)(q[Q - 1] + 1), kEdge);
-      if (KT::kFloat) cb((K)(highest + 1), kNan);
-      if (!emit) share = m_int ? (a.B / m_int > 0 ? a.B / m_int : 1) : 1;
+  // Every run of equal pivots [i, j] emits, in pivot order: a one-key range
+  // when it is the last run or a repeated value (then also the interior range
+  // above it when the gap to the next pivot holds two or more keys), else one
+  // interior range.  Runs are found in parallel (a run starts where the pivot
+  // changes) and placed by a block scan; NaN and edge ranges bracket them.
+  __shared__ uint32_t scan_tmp[32];
+  {
+    enum : uint32_t { kSingle = 0, kEdge = 1, kInterior = 2, kNan = 3 };
+    const uint32_t pre = (KT::kFloat ? 1u : 0u) + (q[0] > lowest ? 1u : 0u);
+    const uint32_t per_q = (Q + blockDim.x - 1) / blockDim.x;
+    const uint32_t i0 = threadIdx.x * per_q, i1 = i0 + per_q < Q ? i0 + per_q : Q;
+    uint32_t cnt = 0, nint = 0;
+    for (uint32_t i = i0; i < i1; ++i) {
+      if (i > 0 && q[i] == q[i - 1]) continue;
+      uint32_t j = i;
+      while (j + 1 < Q && q[j + 1] == q[i]) ++j;
+      if (j + 1 >= Q) cnt += 1;
+      else if (j > i) {
+        cnt += 1;
+        if ((K)(q[j + 1] - q[i]) >= 2) { cnt += 1; nint += 1; }
+      } else { cnt += 1; nint += 1; }
     }
-    bcast_R = R;
+    uint32_t total, ints;
+    uint32_t at = pre + block_excl_scan_u32(cnt, scan_tmp, &total);
+    (void)block_excl_scan_u32(nint, scan_tmp, &ints);
+    const uint32_t share = ints ? (a.B / ints > 0 ? a.B / ints : 1u) : 1u;
+    for (uint32_t i = i0; i < i1; ++i) {
+      if (i > 0 && q[i] == q[i - 1]) continue;
+      const K v = q[i];
+      uint32_t j = i;
+      while (j + 1 < Q && q[j + 1] == v) ++j;
+      if (j + 1 >= Q) {
+        rlo[at] = v; rreq[at++] = 1;
+      } else if (j > i) {
+        rlo[at] = v; rreq[at++] = 1;
+        if ((K)(q[j + 1] - v) >= 2) { rlo[at] = (K)(v + 1); rreq[at++] = share; }
+      } else {
+        rlo[at] = v; rreq[at++] = share;
+      }
+    }
+    if (threadIdx.x == 0) {
+      uint32_t r = 0;
+      if (KT::kFloat) { rlo[r] = (K)0; rreq[r++] = 1; }
+      if (q[0] > lowest) { rlo[r] = lowest; rreq[r++] = a.E; }
+      r = pre + total;
+      if (q[Q - 1] < highest) { rlo[r] = (K)(q[Q - 1] + 1); rreq[r++] = a.E; }
+      if (KT::kFloat) { rlo[r] = (K)(highest + 1); rreq[r++] = 1; }
+      bcast_R = r;
+    }
+    (void)kSingle; (void)kEdge; (void)kInterior; (void)kNan;
   }
   __syncthreads();
   const uint32_t R = bcast_R;
   // 4. map parameters (parallel), bucket bases (block scan), global copy
-  __shared__ uint32_t scan_tmp[32];
   uint32_t per = (R + blockDim.x - 1) / blockDim.x;
   uint32_t r0 = threadIdx.x * per, r1 = r0 + per < R ? r0 + per : R;
   uint32_t mysum = 0;
