@@ -1127,6 +1127,15 @@ template <class K> struct Pass1Map {
     uint32_t s0 = slot0[m0.bucket(key)];
     return param_bucket<K>(p1[s0], key);
   }
+  template <int N> __device__ __forceinline__ void buckets(const K (&key)[N], uint32_t (&b)[N]) const {
+    uint32_t b0[N];
+    m0.template buckets<N>(key, b0);
+    uint32_t s0[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) s0[j] = slot0[b0[j]];
+#pragma unroll
+    for (int j = 0; j < N; ++j) b[j] = param_bucket<K>(p1[s0[j]], key[j]);
+  }
 };
 
 // LDS bytes of the pass-1 map; R1 = 0 when the parameters stay in global memory.
@@ -1171,7 +1180,15 @@ __global__ void __launch_bounds__(1024) k_count1(const K* __restrict__ buf, cons
   __syncthreads();
   uint32_t t0 = blockIdx.x * tile;
   uint32_t t1 = t0 + tile < n ? t0 + tile : n;
-  stream_tile<K>(buf, t0, t1, [&](K key, uint32_t) { lds_count(hist, pm.bucket(key)); });
+  constexpr int N = Vec16<K>::N;
+  stream_tile_vec<K>(
+      buf, t0, t1,
+      [&](const K (&k)[N]) {
+        uint32_t b[N];
+        pm.template buckets<N>(k, b);
+        lds_count_n<N>(hist, b);
+      },
+      [&](K key) { lds_count(hist, pm.bucket(key)); });
   __syncthreads();
   uint32_t* row = rows + (size_t)blockIdx.x * B;
   for (uint32_t i = threadIdx.x; i < B; i += blockDim.x) row[i] = hist[i];
@@ -1196,12 +1213,28 @@ __global__ void __launch_bounds__(1024) k_compact1(const K* __restrict__ buf, co
   __syncthreads();
   uint32_t t0 = blockIdx.x * tile;
   uint32_t t1 = t0 + tile < n ? t0 + tile : n;
-  stream_tile<K>(buf, t0, t1, [&](K key, uint32_t) {
-    uint32_t s = ct.slot[pm.bucket(key)];
-    bool act = s != kNone16;
-    uint32_t p = lds_claim(ct.cursor, s, act);
-    if (act) out[p] = key;
-  });
+  constexpr int N = Vec16<K>::N;
+  stream_tile_vec<K>(
+      buf, t0, t1,
+      [&](const K (&k)[N]) {
+        uint32_t b[N];
+        pm.template buckets<N>(k, b);
+        uint32_t sl[N];
+#pragma unroll
+        for (int j = 0; j < N; ++j) sl[j] = ct.slot[b[j]];
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+          bool act = sl[j] != kNone16;
+          uint32_t p = lds_claim(ct.cursor, sl[j], act);
+          if (act) out[p] = k[j];
+        }
+      },
+      [&](K key) {
+        uint32_t s = ct.slot[pm.bucket(key)];
+        bool act = s != kNone16;
+        uint32_t p = lds_claim(ct.cursor, s, act);
+        if (act) out[p] = key;
+      });
 }
 
 template <class K>

@@ -1,0 +1,88 @@
+#include "hip/hip_runtime.h"
+#include <algorithm>
+#include <random>
+#include "splitter.hpp"
+thread_local dim3 threadIdx;
+dim3 blockIdx, blockDim, gridDim;
+EmuBarrier g_bar;
+unsigned char* g_dyn_smem;
+struct WaveSync { std::mutex m; std::condition_variable cv; uint32_t arrived = 0, gen = 0; uint32_t v[64]; uint32_t out[64]; };
+static WaveSync g_w[16];
+uint32_t emu_wave_incl_scan(uint32_t v) {
+  uint32_t w = threadIdx.x >> 6, l = threadIdx.x & 63; WaveSync& s = g_w[w];
+  std::unique_lock<std::mutex> lk(s.m); uint32_t g = s.gen; s.v[l] = v;
+  if (++s.arrived == 64) { uint32_t acc = 0; for (int i = 0; i < 64; ++i) { acc += s.v[i]; s.out[i] = acc; } s.arrived = 0; s.gen++; s.cv.notify_all(); }
+  else s.cv.wait(lk, [&] { return s.gen != g; });
+  return s.out[l];
+}
+// shfl_xor among 32-lane groups: emulate via a per-wave exchange (all lanes call)
+struct XSync { std::mutex m; std::condition_variable cv; uint32_t arrived = 0, gen = 0; float v[64]; };
+static XSync g_x[16];
+float __shfl_xor(float v, int mask, int width) {
+  uint32_t w = threadIdx.x >> 6, l = threadIdx.x & 63; XSync& s = g_x[w];
+  std::unique_lock<std::mutex> lk(s.m);
+  uint32_t g = s.gen; s.v[l] = v;
+  if (++s.arrived == 64) { s.arrived = 0; s.gen++; s.cv.notify_all(); } else s.cv.wait(lk, [&] { return s.gen != g; });
+  float r = s.v[(l & ~(uint32_t)(width - 1)) | ((l ^ mask) & (width - 1))];
+  // second phase so nobody overwrites v before all have read
+  g = s.gen;
+  if (++s.arrived == 64) { s.arrived = 0; s.gen++; s.cv.notify_all(); } else s.cv.wait(lk, [&] { return s.gen != g; });
+  return r;
+}
+using namespace ggms;
+int run(uint32_t mode, uint32_t S, uint32_t P, uint32_t B, int kind, uint64_t seed) {
+  std::mt19937_64 rng(seed);
+  std::normal_distribution<float> nd;
+  uint32_t n = 1 << 20;
+  std::vector<float> x(n);
+  for (auto& v : x) { float f = nd(rng); if (kind == 1) f = (rng() % 10 < 9) ? 2.5f : f; if (kind == 2) f = (float)(rng() % 5); x[&v - x.data()] = f; }
+  std::vector<float> w(kMlpParams);
+  for (auto& v : w) v = (float)((int)(rng() % 2001) - 1000) / 1000.0f;
+  Plan0Args a{}; a.mode = mode; a.S = S; a.P = P; a.B = B; a.E = 4; a.strideS = 8192; a.seed = seed; a.mlp = w.data();
+  std::vector<Range<uint32_t>> ranges(4096);
+  std::vector<uint32_t> lut(LutGeom<uint32_t>::L1CELLS + LutGeom<uint32_t>::L2TABLES * 64);
+  PassCtl ctl{}; ctl.n = n;
+  uint32_t Sp = 1; while (Sp < S) Sp <<= 1;
+  std::vector<unsigned char> smem(plan0_lds_bytes<uint32_t>(Sp) + 64);
+  g_dyn_smem = smem.data();
+  blockDim = dim3(1024); g_bar.reset(1024);
+  std::vector<std::thread> th;
+  for (uint32_t t = 0; t < 1024; ++t) th.emplace_back([&, t] { threadIdx = dim3(t); k_plan0<float>(x.data(), a, ranges.data(), lut.data(), &ctl); g_bar.exit_thread(); });
+  for (auto& t : th) t.join();
+  // check ranges partition key space and LUT maps each key to the right range
+  uint32_t R = ctl.R, bad = 0, base = 0;
+  if (ranges[0].lo != 0) bad++;
+  for (uint32_t r = 0; r < R; ++r) {
+    if (r + 1 < R && !(ranges[r].lo < ranges[r + 1].lo)) bad++;
+    if (r + 1 < R && ranges[r].hi != ranges[r + 1].lo - 1) bad++;
+    if (ranges[r].base != base) bad++;
+    base += ranges[r].nb;
+  }
+  if (base != ctl.B) bad++;
+  // reference map by binary search vs LUT-based map
+  std::vector<uint32_t> starts(R); for (uint32_t r = 0; r < R; ++r) starts[r] = ranges[r].lo;
+  std::vector<MapParam<uint32_t>> prm(R);
+  for (uint32_t r = 0; r < R; ++r) { prm[r].lo = ranges[r].lo; prm[r].base = ranges[r].base; prm[r].mult = ranges[r].mult; prm[r].sh = ranges[r].sh; }
+  Pass0Map<uint32_t> mp; mp.lut = lut.data(); mp.prm = prm.data();
+  for (int t = 0; t < 200000; ++t) {
+    uint32_t key = (uint32_t)rng();
+    if (t % 2) key = KeyTraits<float>::enc(x[rng() % n]);
+    uint32_t r = std::upper_bound(starts.begin(), starts.end(), key) - starts.begin() - 1;
+    uint32_t want = ranges[r].base + map_off<uint32_t>(key, ranges[r].lo, ranges[r].sh, ranges[r].mult);
+    threadIdx = dim3(0);
+    uint32_t got = mp.bucket(key);
+    if (got != want) { if (bad < 5) printf("key %08x got %u want %u (r=%u)\n", key, got, want, r); bad++; }
+  }
+  printf("mode %u S %u P %u kind %d: R=%u B=%u %s\n", mode, S, P, kind, R, ctl.B, bad ? "FAIL" : "ok");
+  return bad ? 1 : 0;
+}
+int main() {
+  int bad = 0;
+  for (int kind = 0; kind < 3; ++kind) {
+    bad += run(kModeLinear, 1024, 17, 8192, kind, 1 + kind);
+    bad += run(kModeSketch, 8192, 256, 16384, kind, 11 + kind);
+    bad += run(kModeLearned, 8192, 256, 16384, kind, 21 + kind);
+    bad += run(kModeLearned, 8192, 512, 16384, kind, 31 + kind);
+  }
+  return bad;
+}

@@ -1,0 +1,40 @@
+#include "hip/hip_runtime.h"
+#include <algorithm>
+#include <random>
+#include "select_kernels.hpp"
+thread_local dim3 threadIdx;
+dim3 blockIdx, blockDim, gridDim;
+EmuBarrier g_bar;
+unsigned char* g_dyn_smem;
+struct WaveSync { std::mutex m; std::condition_variable cv; uint32_t arrived = 0, gen = 0; uint32_t v[64]; uint32_t out[64]; };
+static WaveSync g_w[16];
+uint32_t emu_wave_incl_scan(uint32_t v) {
+  uint32_t w = threadIdx.x >> 6, l = threadIdx.x & 63; WaveSync& s = g_w[w];
+  std::unique_lock<std::mutex> lk(s.m); uint32_t g = s.gen; s.v[l] = v;
+  if (++s.arrived == 64) { uint32_t acc = 0; for (int i = 0; i < 64; ++i) { acc += s.v[i]; s.out[i] = acc; } s.arrived = 0; s.gen++; s.cv.notify_all(); }
+  else s.cv.wait(lk, [&] { return s.gen != g; });
+  return s.out[l];
+}
+template <class K> int run(uint32_t M, uint32_t nt, uint64_t seed) {
+  std::mt19937_64 rng(seed);
+  std::vector<K> a(M);
+  for (auto& v : a) v = (K)(rng() % (seed % 3 == 0 ? 7 : ~0ull));
+  std::vector<K> want = a; std::sort(want.begin(), want.end());
+  blockDim = dim3(nt); g_bar.reset(nt);
+  std::vector<std::thread> th;
+  for (uint32_t t = 0; t < nt; ++t) th.emplace_back([&, t] { threadIdx = dim3(t); ggms::bitonic_sort_lds<K>(a.data(), M); g_bar.exit_thread(); });
+  for (auto& x : th) x.join();
+  if (a != want) { printf("FAIL M=%u nt=%u K=%zu\n", M, nt, sizeof(K)); return 1; }
+  return 0;
+}
+int main() {
+  int bad = 0;
+  for (uint32_t nt : {256u, 1024u})
+    for (uint32_t M = 1; M <= 16384; M <<= 1) {
+      bad += run<uint32_t>(M, nt, M + nt);
+      bad += run<uint32_t>(M, nt, 3 * (M + nt));
+      if (M <= 8192) bad += run<uint64_t>(M, nt, M * 7 + nt);
+    }
+  printf(bad ? "sort FAIL\n" : "sort ok\n");
+  return bad;
+}
