@@ -368,7 +368,7 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
     prof.begin(GGMS_KERNEL_FINISH, 0);
     k_finish<T><<<1, 1024, fin_lds, st>>>(bufA, ctl, (uint32_t*)P(bSmallStart), (uint32_t*)P(bSmallLen),
                                           (K*)P(bSmallKlo), (K*)P(bSmallKhi), (uint32_t*)P(bSmallKStart), klist, k_rank, k_state, d_out,
-                                          c.fin_cap, &ctl->pad[0]);
+                                          0u, c.fin_cap, c.fin_cap, &ctl->pad[0]);
     prof.end();
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(&hctl[0], ctl, ctlb, hipMemcpyDeviceToHost, st));
@@ -523,10 +523,17 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
       }
       HIP_TRY(hipGetLastError());
       if (pass > 0) {  // the first pass hands every k-bucket to pass 1 (no small segments yet)
+      // two size classes: segments up to half the capacity with half the LDS
+      // (two workgroups per CU), then the rest with the full buffer
+      prof.begin(GGMS_KERNEL_FINISH, (int)pass);
+      k_finish<T><<<2 * fin_grid, 1024, fin_lds / 2, st>>>(dst, cp, pl.small_start, pl.small_len, pl.small_klo,
+                                                           pl.small_khi, pl.small_kstart, klist, k_rank, k_state,
+                                                           d_out, 0u, c.fin_cap / 2, c.fin_cap, &cp->pad[0]);
+      prof.end();
       prof.begin(GGMS_KERNEL_FINISH, (int)pass);
       k_finish<T><<<fin_grid, 1024, fin_lds, st>>>(dst, cp, pl.small_start, pl.small_len, pl.small_klo,
-                                                   pl.small_khi, pl.small_kstart, klist,
-                                                   k_rank, k_state, d_out, c.fin_cap, &cp->pad[0]);
+                                                   pl.small_khi, pl.small_kstart, klist, k_rank, k_state, d_out,
+                                                   c.fin_cap / 2, c.fin_cap, c.fin_cap, &cp->pad[0]);
       prof.end();
       HIP_TRY(hipGetLastError());
       }

@@ -1265,6 +1265,8 @@ __global__ void __launch_bounds__(1024) k_compactS(const K* __restrict__ buf, co
 
 // ---------------------------------------------------------------------------
 // k_finish: exact answers for the ks of every small segment.
+// Segments are split into size classes (len_lo, len_hi] so that the common
+// short segments run with half the LDS and two workgroups per CU.
 // The segment (<= fin_cap keys) is loaded into LDS once.  A segment holding
 // few ks answers each by radix select: 8-bit digits over only the low bits in
 // which the segment's key interval [klo, khi] varies (all keys share the
@@ -1323,7 +1325,8 @@ __global__ void __launch_bounds__(1024) k_finish(const typename KeyTraits<T>::K*
                                                  const uint32_t* __restrict__ small_kstart,
                                                  const uint32_t* __restrict__ klist, uint32_t* __restrict__ k_rank,
                                                  uint32_t* __restrict__ k_state, T* __restrict__ out,
-                                                 uint32_t fin_cap, uint32_t* __restrict__ err) {
+                                                 uint32_t len_lo, uint32_t len_hi, uint32_t fin_cap,
+                                                 uint32_t* __restrict__ err) {
   using K = typename KeyTraits<T>::K;
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ uint32_t hist[256];
@@ -1333,9 +1336,10 @@ __global__ void __launch_bounds__(1024) k_finish(const typename KeyTraits<T>::K*
   for (uint32_t j = blockIdx.x; j < U_small; j += gridDim.x) {
     uint32_t st = small_start[j], len = small_len[j];
     if (len > fin_cap) {  // cannot happen unless the pass cap forced finishing; reported, never sorted
-      if (threadIdx.x == 0) atomicAdd(err, 1u);
+      if (threadIdx.x == 0 && len_hi == fin_cap) atomicAdd(err, 1u);
       continue;
     }
+    if (len <= len_lo || len > len_hi) continue;  // another launch's size class
     uint32_t q0 = small_kstart[j], q1 = small_kstart[j + 1];
     if (q1 - q0 <= kSelectMaxKs) {
       for (uint32_t i = threadIdx.x; i < len; i += blockDim.x) s[i] = buf[st + i];
