@@ -320,16 +320,14 @@ __device__ __forceinline__ void stream_tile_vec_any(const V* __restrict__ x, uin
 }
 
 // hist[b[j]] += 1 for N buckets per lane (every lane of the wave active for
-// all N).  One compare + ballot per bucket decides whether any of the N
-// needs duplicate peeling; the common case is N plain LDS atomics.
+// all N).  One compare + ballot decides whether the group needs duplicate
+// peeling; the common case is N plain LDS atomics.
 template <int N>
 __device__ __forceinline__ void lds_count_n(uint32_t* hist, const uint32_t (&b)[N]) {
-  bool peel = false;
-#pragma unroll
-  for (int j = 0; j < N; ++j) {
-    uint32_t b0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)b[j]);
-    peel |= __builtin_popcountll(wave_ballot(b[j] == b0)) >= kPeelMin;
-  }
+  // a value dominant enough to need peeling shows up in the first key of the
+  // group as well; the other keys of the group are not checked
+  const uint32_t b0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)b[0]);
+  const bool peel = __builtin_popcountll(wave_ballot(b[0] == b0)) >= kPeelMin;
   if (peel) {
 #pragma unroll
     for (int j = 0; j < N; ++j) lds_count(hist, b[j]);
@@ -429,9 +427,18 @@ template <class K> struct Pass0Map {
     for (uint32_t i = threadIdx.x; i < G::L1CELLS + G::L2TABLES * 64; i += blockDim.x) lut[i] = g_lut[i];
     for (uint32_t i = threadIdx.x; i < R; i += blockDim.x) prm[i] = to_param<K>(g[i]);
   }
+  // Range of a key in a level-1 leaf entry: r0 (the bits above S1; the flag
+  // bit is 0 for a leaf) plus one when the key's in-cell offset passes the
+  // stored start offset.  Splicing the key's low bits under the entry's high
+  // bits turns that test into one compare of whole words.  Meaningless for
+  // non-leaf entries, which the callers overwrite.
+  static __device__ __forceinline__ uint32_t leaf_range(K e, K key) {
+    const K spliced = (key & G::M1) | (e & (K)~G::M1);
+    return (uint32_t)(e >> G::S1) + (spliced > e ? 1u : 0u);
+  }
   __device__ __forceinline__ uint32_t range_of(K key) const {
     K e = lut[(uint32_t)(key >> G::S1)];
-    uint32_t r = (uint32_t)((e >> G::S1) & 0x7FF) + ((key & G::M1) > (e & G::M1) ? 1u : 0u);
+    uint32_t r = leaf_range(e, key);
     bool nl = (e & G::FLAG) != 0;
     if (wave_ballot(nl)) {
       if (nl) {
@@ -463,7 +470,7 @@ template <class K> struct Pass0Map {
     bool nl = false;
 #pragma unroll
     for (int j = 0; j < N; ++j) {
-      r[j] = (uint32_t)((e[j] >> G::S1) & 0x7FF) + ((key[j] & G::M1) > (e[j] & G::M1) ? 1u : 0u);
+      r[j] = leaf_range(e[j], key[j]);
       nl |= (e[j] & G::FLAG) != 0;
     }
     if (wave_ballot(nl)) {
@@ -591,9 +598,10 @@ __global__ void __launch_bounds__(1024) k_count0(const T* __restrict__ x, const 
   extern __shared__ __align__(16) unsigned char smem[];
   const uint32_t B = ctl->B, n = ctl->n, tile = ctl->tile;
   if (blockIdx.x >= ctl->G) return;
-  uint32_t* hist = reinterpret_cast<uint32_t*>(smem);
+  // the cell table sits at LDS offset 0 so a cell address is just the shifted key
   Pass0Map<K> mp;
-  mp.carve(smem + ((size_t)B * 4 + 15) / 16 * 16);
+  mp.carve(smem);
+  uint32_t* hist = reinterpret_cast<uint32_t*>(smem + (Pass0Map<K>::bytes(ctl->R) + 15) / 16 * 16);
   for (uint32_t i = threadIdx.x; i < B; i += blockDim.x) hist[i] = 0;
   mp.load(lut, ranges, ctl->R);
   __syncthreads();
@@ -1055,9 +1063,9 @@ __global__ void __launch_bounds__(1024) k_compact0(const T* __restrict__ x, cons
   __shared__ uint32_t part[16];
   const uint32_t B = ctl->B, n = ctl->n, tile = ctl->tile, U_slots = ctl->U_slots;
   if (blockIdx.x >= ctl->G || U_slots == 0) return;
-  uint16_t* slot = reinterpret_cast<uint16_t*>(smem);
   Pass0Map<K> mp;
-  mp.carve(smem + ((size_t)B * 2 + 15) / 16 * 16);
+  mp.carve(smem);
+  uint16_t* slot = reinterpret_cast<uint16_t*>(smem + (Pass0Map<K>::bytes(ctl->R) + 15) / 16 * 16);
   for (uint32_t i = threadIdx.x; i < B; i += blockDim.x) slot[i] = slot_of_bucket[i];
   mp.load(lut, ranges, ctl->R);
   const uint32_t* rp = rows + (size_t)blockIdx.x * B;
@@ -1141,7 +1149,7 @@ template <class K> struct Pass1Map {
 // LDS bytes of the pass-1 map; R1 = 0 when the parameters stay in global memory.
 template <class K>
 __host__ __device__ __forceinline__ size_t pass1_map_bytes(uint32_t B0, uint32_t R0, uint32_t R1) {
-  return ((size_t)B0 * 2 + 15) / 16 * 16 + Pass0Map<K>::bytes(R0) + (size_t)R1 * sizeof(MapParam<K>);
+  return Pass0Map<K>::bytes(R0) + ((size_t)B0 * 2 + 15) / 16 * 16 + (size_t)R1 * sizeof(MapParam<K>);
 }
 
 template <class K>
@@ -1149,14 +1157,14 @@ __device__ __forceinline__ void pass1_map_init(Pass1Map<K>& pm, unsigned char* p
                                                const K* lut, const Range<K>* rng0, const uint16_t* slot0_g,
                                                const MapParam<K>* p1_g, uint32_t R1, bool p1_in_lds) {
   const uint32_t B0 = ctl0->B, R0 = ctl0->R;
-  uint16_t* s0 = reinterpret_cast<uint16_t*>(p);
+  pm.m0.carve(p);  // first-pass cell table first (p is LDS offset 0 in the pass-1 kernels)
+  pm.m0.load(lut, rng0, R0);
+  uint16_t* s0 = reinterpret_cast<uint16_t*>(p + Pass0Map<K>::bytes(R0));
   for (uint32_t i = threadIdx.x; i < B0; i += blockDim.x) s0[i] = slot0_g[i];
   pm.slot0 = s0;
-  unsigned char* q = p + ((size_t)B0 * 2 + 15) / 16 * 16;
-  pm.m0.carve(q);
-  pm.m0.load(lut, rng0, R0);
+  unsigned char* q = p + Pass0Map<K>::bytes(R0) + ((size_t)B0 * 2 + 15) / 16 * 16;
   if (p1_in_lds) {
-    MapParam<K>* p1 = reinterpret_cast<MapParam<K>*>(q + Pass0Map<K>::bytes(R0));
+    MapParam<K>* p1 = reinterpret_cast<MapParam<K>*>(q);
     for (uint32_t i = threadIdx.x; i < R1; i += blockDim.x) p1[i] = p1_g[i];
     pm.p1 = p1;
   } else {
@@ -1173,9 +1181,10 @@ __global__ void __launch_bounds__(1024) k_count1(const K* __restrict__ buf, cons
   extern __shared__ __align__(16) unsigned char smem[];
   const uint32_t B = ctl->B, n = ctl->n, tile = ctl->tile;
   if (blockIdx.x >= ctl->G) return;
-  uint32_t* hist = reinterpret_cast<uint32_t*>(smem);
   Pass1Map<K> pm;
-  pass1_map_init<K>(pm, smem + ((size_t)B * 4 + 15) / 16 * 16, ctl0, lut, rng0, slot0, p1, ctl->R, p1_in_lds != 0);
+  pass1_map_init<K>(pm, smem, ctl0, lut, rng0, slot0, p1, ctl->R, p1_in_lds != 0);
+  uint32_t* hist = reinterpret_cast<uint32_t*>(
+      smem + pass1_map_bytes<K>(ctl0->B, ctl0->R, p1_in_lds ? ctl->R : 0));
   for (uint32_t i = threadIdx.x; i < B; i += blockDim.x) hist[i] = 0;
   __syncthreads();
   uint32_t t0 = blockIdx.x * tile;
@@ -1206,10 +1215,11 @@ __global__ void __launch_bounds__(1024) k_compact1(const K* __restrict__ buf, co
   extern __shared__ __align__(16) unsigned char smem[];
   const uint32_t B = ctl->B, n = ctl->n, tile = ctl->tile, U_slots = ctl->U_slots;
   if (blockIdx.x >= ctl->G || U_slots == 0) return;
-  CompactTables ct;
-  compact_init(ct, smem, B, U_slots, slot_of_bucket, slot_dst, slot_bucket, rows + (size_t)blockIdx.x * B);
   Pass1Map<K> pm;
-  pass1_map_init<K>(pm, smem + compact_lds_bytes(B, U_slots + 4), ctl0, lut, rng0, slot0, p1, ctl->R, p1_in_lds != 0);
+  pass1_map_init<K>(pm, smem, ctl0, lut, rng0, slot0, p1, ctl->R, p1_in_lds != 0);
+  CompactTables ct;
+  compact_init(ct, smem + pass1_map_bytes<K>(ctl0->B, ctl0->R, p1_in_lds ? ctl->R : 0), B, U_slots, slot_of_bucket,
+               slot_dst, slot_bucket, rows + (size_t)blockIdx.x * B);
   __syncthreads();
   uint32_t t0 = blockIdx.x * tile;
   uint32_t t1 = t0 + tile < n ? t0 + tile : n;
