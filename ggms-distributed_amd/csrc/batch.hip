@@ -551,8 +551,16 @@ __global__ void __launch_bounds__(1024) k_batch(BatchArgs a) {
         const uint32_t nb = rr.nb;
         for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) L.H[i] = 0;
         __syncthreads();
-        stream_tile_any<K>(src, 0, it.len,
-                           [&](K key, uint32_t) { lds_count(L.H, map_off<K>(key, rr.lo, rr.sh, rr.mult)); });
+        constexpr int NK = Vec16<K>::N;
+        stream_tile_vec_any<K>(
+            src, 0, it.len,
+            [&](const K (&k)[NK]) {
+              uint32_t b[NK];
+#pragma unroll
+              for (int j = 0; j < NK; ++j) b[j] = map_off<K>(k[j], rr.lo, rr.sh, rr.mult);
+              lds_count_n<NK>(L.H, b);
+            },
+            [&](K key) { lds_count(L.H, map_off<K>(key, rr.lo, rr.sh, rr.mult)); });
         __syncthreads();
         block_incl_scan_lds(L.H, nb, scan_s);
         auto interval1 = [&](uint32_t b, K& klo, K& khi) { bucket_interval<K>(rr, b, klo, khi); };
@@ -562,12 +570,19 @@ __global__ void __launch_bounds__(1024) k_batch(BatchArgs a) {
         const uint32_t nk2 = s_nkids;
         K* dst = (it.buf ? sA : sB);
         if (nk2) {
-          stream_tile_any<K>(src, 0, it.len, [&](K key, uint32_t) {
+          auto claim = [&](K key) {
             uint32_t s = slot1[map_off<K>(key, rr.lo, rr.sh, rr.mult)];
             bool act = s != kNone16;
             uint32_t pos = lds_claim(L.cursor, s, act);
             if (act) dst[pos] = key;
-          });
+          };
+          stream_tile_vec_any<K>(
+              src, 0, it.len,
+              [&](const K (&k)[NK]) {
+#pragma unroll
+                for (int j = 0; j < NK; ++j) claim(k[j]);
+              },
+              claim);
           __threadfence_block();
         }
         next_sp += nk2;

@@ -309,14 +309,21 @@ __device__ __forceinline__ void stream_tile_vec(const V* __restrict__ x, uint32_
   for (uint32_t i = tail0 + threadIdx.x; i < t1; i += nt) f1(x[i]);
 }
 
+// Any start: the elements before the first 16-byte boundary go one per lane,
+// the rest through the vector path.
 template <class V, class FV, class F1>
 __device__ __forceinline__ void stream_tile_vec_any(const V* __restrict__ x, uint32_t t0, uint32_t t1, FV&& fv,
                                                     F1&& f1) {
-  if ((reinterpret_cast<uintptr_t>(x) & 15u) == 0) {
-    stream_tile_vec<V>(x, t0, t1, fv, f1);
-  } else {
+  if (t0 >= t1) return;
+  const uint32_t mis = (uint32_t)(reinterpret_cast<uintptr_t>(x + t0) & 15u);
+  if (mis % sizeof(V) != 0) {  // not even element-aligned to a 16-byte grid: scalar only
     for (uint32_t i = t0 + threadIdx.x; i < t1; i += blockDim.x) f1(x[i]);
+    return;
   }
+  uint32_t head = mis ? (16u - mis) / (uint32_t)sizeof(V) : 0u;
+  if (head > t1 - t0) head = t1 - t0;
+  if (threadIdx.x < head) f1(x[t0 + threadIdx.x]);
+  stream_tile_vec<V>(x + t0 + head, 0, t1 - t0 - head, fv, f1);
 }
 
 // hist[b[j]] += 1 for N buckets per lane (every lane of the wave active for
