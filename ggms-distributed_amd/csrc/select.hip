@@ -78,10 +78,12 @@ static Config resolve_config(ggms_alg alg, uint64_t n, uint32_t nk, const ggms_o
   if (alg == GGMS_ALG_DISTRIBUTED_1GPU) c.mode = kModeStrided;
   if (o.splitter == GGMS_SPLIT_LEARNED) c.mode = kModeLearned;
   if (o.splitter == GGMS_SPLIT_SKETCH) c.mode = kModeSketch;
-  // 0 = automatic: the reference's 8192 for the linear splitter; 16384 for the
-  // sample-quantile splitters, whose near equal-depth buckets then hand most
-  // k-buckets straight to the finishing kernel
-  c.B = o.num_buckets ? o.num_buckets : ((c.mode == kModeLinear || c.mode == kModeStrided) ? 8192u : 16384u);
+  // 0 = automatic: 8192 (the reference's count) for every splitter.  With
+  // 8192 buckets the first pass's histogram and cell table leave room for two
+  // streaming workgroups per CU; 16384 buckets halve the survivors of the
+  // quantile splitters but the single workgroup per CU costs more than that
+  // saves (measured 1.36 against 1.32 ms at 2^28 fp32, 1001 ks).
+  c.B = o.num_buckets ? o.num_buckets : 8192u;
   c.B = std::max<uint32_t>(16, std::min<uint32_t>(c.B, kMaxBuckets));
   c.B_cap = kMaxBuckets;
   if (c.mode == kModeLinear || c.mode == kModeStrided) {
@@ -262,10 +264,17 @@ static Layout make_layout(uint64_t n, uint32_t nk, const Config& c) {
   return L;
 }
 
+// Dynamic LDS of the first-pass count: map parameters, then the histogram (the
+// cell table is the kernel's static LDS array).
 template <class T>
 static size_t count_lds0(const Config& c) {
   using K = typename KeyTraits<T>::K;
-  return ((size_t)c.B0_bound * 4 + 15) / 16 * 16 + Pass0Map<K>::bytes(c.R0_bound);
+  return (size_t)c.R0_bound * sizeof(MapParam<K>) + (size_t)c.B0_bound * 4;
+}
+template <class T>
+static size_t count_lds0_total(const Config& c) {
+  using K = typename KeyTraits<T>::K;
+  return count_lds0<T>(c) + LutGeom<K>::bytes();
 }
 
 // LDS ceiling of one workgroup on gfx950 (160 KB per CU; static LDS of the
@@ -424,7 +433,7 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
       // count
       if (pass == 0) {
         size_t lds = count_lds0<T>(c);
-        LDS_CHECK(lds);
+        LDS_CHECK(count_lds0_total<T>(c));
         set_lds_attr((const void*)k_count0<T>, lds);
         prof.begin(GGMS_KERNEL_COUNT0, (int)pass);
         k_count0<T><<<G, NT, lds, st>>>(d_vec, rc, (K*)P(bLut), cp, rows);
@@ -523,17 +532,10 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
       }
       HIP_TRY(hipGetLastError());
       if (pass > 0) {  // the first pass hands every k-bucket to pass 1 (no small segments yet)
-      // two size classes: segments up to half the capacity with half the LDS
-      // (two workgroups per CU), then the rest with the full buffer
-      prof.begin(GGMS_KERNEL_FINISH, (int)pass);
-      k_finish<T><<<2 * fin_grid, 1024, fin_lds / 2, st>>>(dst, cp, pl.small_start, pl.small_len, pl.small_klo,
-                                                           pl.small_khi, pl.small_kstart, klist, k_rank, k_state,
-                                                           d_out, 0u, c.fin_cap / 2, c.fin_cap, &cp->pad[0]);
-      prof.end();
       prof.begin(GGMS_KERNEL_FINISH, (int)pass);
       k_finish<T><<<fin_grid, 1024, fin_lds, st>>>(dst, cp, pl.small_start, pl.small_len, pl.small_klo,
                                                    pl.small_khi, pl.small_kstart, klist, k_rank, k_state, d_out,
-                                                   c.fin_cap / 2, c.fin_cap, c.fin_cap, &cp->pad[0]);
+                                                   0u, c.fin_cap, c.fin_cap, &cp->pad[0]);
       prof.end();
       HIP_TRY(hipGetLastError());
       }
@@ -628,7 +630,7 @@ static ggms_status run_first_pass(ggms_ctx* ctx, const T* d_vec, uint64_t n64, c
   k_plan0<T><<<1, 1024, lds_p0, st>>>(d_vec, pa, rng, (K*)P(bLut), ctl);
   HIP_TRY(hipGetLastError());
   size_t lds = count_lds0<T>(c);
-  LDS_CHECK(lds);
+  LDS_CHECK(count_lds0_total<T>(c));
   set_lds_attr((const void*)k_count0<T>, lds);
   k_count0<T><<<G0, c.NT, lds, st>>>(d_vec, rng, (K*)P(bLut), ctl, rows);
   HIP_TRY(hipGetLastError());

@@ -399,6 +399,7 @@ template <class K> struct LutGeom {
   static constexpr K M2 = ((K)1 << S2) - 1;
   static __host__ __device__ constexpr size_t bytes() { return (size_t)(L1CELLS + L2TABLES * 64) * sizeof(K); }
   static __host__ __device__ constexpr uint32_t max_ranges() { return 2047; }
+  static constexpr uint32_t kEntries = L1CELLS + L2TABLES * 64;
 };
 
 template <class K> __device__ __forceinline__ MapParam<K> to_param(const Range<K>& r) {
@@ -605,10 +606,13 @@ __global__ void __launch_bounds__(1024) k_count0(const T* __restrict__ x, const 
   extern __shared__ __align__(16) unsigned char smem[];
   const uint32_t B = ctl->B, n = ctl->n, tile = ctl->tile;
   if (blockIdx.x >= ctl->G) return;
-  // the cell table sits at LDS offset 0 so a cell address is just the shifted key
+  // the cell table is a static LDS array: its address is a compile-time
+  // constant folded into every cell read (a dynamic-LDS base costs an add per key)
+  __shared__ __align__(16) K s_lut[LutGeom<K>::kEntries];
   Pass0Map<K> mp;
-  mp.carve(smem);
-  uint32_t* hist = reinterpret_cast<uint32_t*>(smem + (Pass0Map<K>::bytes(ctl->R) + 15) / 16 * 16);
+  mp.lut = s_lut;
+  mp.prm = reinterpret_cast<MapParam<K>*>(smem);
+  uint32_t* hist = reinterpret_cast<uint32_t*>(smem + (size_t)ctl->R * sizeof(MapParam<K>));
   for (uint32_t i = threadIdx.x; i < B; i += blockDim.x) hist[i] = 0;
   mp.load(lut, ranges, ctl->R);
   __syncthreads();
@@ -1282,8 +1286,7 @@ __global__ void __launch_bounds__(1024) k_compactS(const K* __restrict__ buf, co
 
 // ---------------------------------------------------------------------------
 // k_finish: exact answers for the ks of every small segment.
-// Segments are split into size classes (len_lo, len_hi] so that the common
-// short segments run with half the LDS and two workgroups per CU.
+// A launch handles the segments of one size class (len_lo, len_hi].
 // The segment (<= fin_cap keys) is loaded into LDS once.  A segment holding
 // few ks answers each by radix select: 8-bit digits over only the low bits in
 // which the segment's key interval [klo, khi] varies (all keys share the

@@ -25,10 +25,13 @@ from . import batch as B
 DEFAULT_TABLE = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "weights",
                              "batch_tuning.json")
 BUCKETS = (1024, 2048, 4096, 8048)
-SAMPLES = (512, 1024, 2048)
-PIECES = (16, 32, 64)
-REBUCKET = (4096, 8192)
+SAMPLES = (1024, 2048)
+PIECES = (32, 64)
+REBUCKET = (8192,)
 THREADS = (512, 1024)
+# classes up to 14 (<= 16384 values) are finished in LDS without a bucketing
+# pass, so their parameters never matter
+FIRST_TUNED_CLASS = 15
 
 
 def load(path: str = DEFAULT_TABLE):
@@ -117,7 +120,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=DEFAULT_TABLE)
     ap.add_argument("--problems", type=int, default=512)
-    ap.add_argument("--lo", type=int, default=10)
+    ap.add_argument("--lo", type=int, default=FIRST_TUNED_CLASS)
     ap.add_argument("--hi", type=int, default=22)
     args = ap.parse_args()
     import torch
