@@ -505,8 +505,9 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
       HIP_TRY(hipGetLastError());
       // compaction
       if (pass == 0) {
-        size_t lds = ((size_t)B_bound * 2 + 15) / 16 * 16 + Pass0Map<K>::bytes(R_bound);
-        LDS_CHECK(lds);
+        // dynamic part: map parameters then the slot table (cell table is static LDS)
+        size_t lds = (size_t)R_bound * sizeof(MapParam<K>) + (size_t)B_bound * 2;
+        LDS_CHECK(lds + LutGeom<K>::bytes());
         set_lds_attr((const void*)k_compact0<T>, lds);
         prof.begin(GGMS_KERNEL_COMPACT0, (int)pass);
         k_compact0<T><<<G, NT, lds, st>>>(d_vec, rc, (K*)P(bLut), cp, rows, pl.slot_of_bucket, pl.slot_bucket,

@@ -358,7 +358,8 @@ __device__ __forceinline__ void stream_tile_any(const V* __restrict__ x, uint32_
 // ---------------------------------------------------------------------------
 // First-pass map in LDS: a two-level key-prefix table.
 //
-// Level 1 has 4096 cells (top 12 key bits).  A cell whose key span contains at
+// Level 1 has 8192 cells (top 13 key bits) for 32-bit keys, 4096 (top 12)
+// for 64-bit keys.  A cell whose key span contains at
 // most one range start is a leaf: its first range r0 and the offset of that
 // start inside the cell, so the range of a key is
 //     r0 + ((key & cell_mask) > start_offset - 1)
@@ -369,7 +370,7 @@ __device__ __forceinline__ void stream_tile_any(const V* __restrict__ x, uint32_
 // short binary search.  Both rarer paths sit behind wave-uniform tests.
 //
 // Entry layout (E = K: 32-bit entries for u32 keys, 64-bit for u64 keys;
-// KB = key bits, S1 = KB - 12, S2 = S1 - 6):
+// KB = key bits, S1 = KB - level-1 bits, S2 = S1 - 6):
 //   level-1 leaf:   bit KB-1 = 0 | r0 in bits [S1, S1+11) | start_offset-1 in bits [0, S1)
 //   level-2 leaf:   bit KB-1 = 0 | r0 in bits [S2, S2+17) | start_offset-1 in bits [0, S2)
 //   level-1 table:  bit KB-1 = 1, bit KB-2 = 1 | table index
@@ -388,11 +389,15 @@ template <> struct __align__(16) MapParam<uint64_t> {
 
 template <class K> struct LutGeom {
   static constexpr uint32_t KB = sizeof(K) * 8;
-  static constexpr uint32_t L1BITS = 12;
+  // 13 level-1 bits for u32 keys: with up to ~128 quantile ranges over a
+  // normal-like distribution every cell holding real mass is a leaf (at 12
+  // bits, 60 % of the mass of a 128-range list sits in two-start cells and
+  // takes the level-2 path).  u64 keys keep 12 bits (the table is 8-byte).
+  static constexpr uint32_t L1BITS = sizeof(K) == 4 ? 13u : 12u;
   static constexpr uint32_t L1CELLS = 1u << L1BITS;
   static constexpr uint32_t S1 = KB - L1BITS;
   static constexpr uint32_t S2 = S1 - 6;
-  static constexpr uint32_t L2TABLES = sizeof(K) == 4 ? 64u : 32u;  // 16 KB of level-2 either way
+  static constexpr uint32_t L2TABLES = sizeof(K) == 4 ? 16u : 32u;  // 4 KB (u32) / 16 KB (u64) of level 2
   static constexpr K FLAG = (K)1 << (KB - 1);
   static constexpr K TABLE = (K)1 << (KB - 2);
   static constexpr K M1 = ((K)1 << S1) - 1;
@@ -1074,9 +1079,11 @@ __global__ void __launch_bounds__(1024) k_compact0(const T* __restrict__ x, cons
   __shared__ uint32_t part[16];
   const uint32_t B = ctl->B, n = ctl->n, tile = ctl->tile, U_slots = ctl->U_slots;
   if (blockIdx.x >= ctl->G || U_slots == 0) return;
+  __shared__ __align__(16) K s_lut[LutGeom<K>::kEntries];  // static: folded cell addresses
   Pass0Map<K> mp;
-  mp.carve(smem);
-  uint16_t* slot = reinterpret_cast<uint16_t*>(smem + (Pass0Map<K>::bytes(ctl->R) + 15) / 16 * 16);
+  mp.lut = s_lut;
+  mp.prm = reinterpret_cast<MapParam<K>*>(smem);
+  uint16_t* slot = reinterpret_cast<uint16_t*>(smem + (size_t)ctl->R * sizeof(MapParam<K>));
   for (uint32_t i = threadIdx.x; i < B; i += blockDim.x) slot[i] = slot_of_bucket[i];
   mp.load(lut, ranges, ctl->R);
   const uint32_t* rp = rows + (size_t)blockIdx.x * B;
