@@ -91,7 +91,7 @@ static Config resolve_config(ggms_alg alg, uint64_t n, uint32_t nk, const ggms_o
     c.P = std::max<uint32_t>(2, std::min<uint32_t>(c.P, 1024));
     c.S = o.sample_size ? o.sample_size : 1024;
   } else {
-    c.P = o.learned_pieces ? o.learned_pieces : 256;
+    c.P = o.learned_pieces ? o.learned_pieces : 64;
     c.P = std::max<uint32_t>(2, std::min<uint32_t>(c.P, kMaxPieces));
     c.S = o.sample_size ? o.sample_size : 8192;
     c.S = std::max<uint32_t>(c.S, 2 * kMlpWin);
@@ -710,7 +710,7 @@ void ggms_opts_default(ggms_opts* o) {
   o->block_threads = 1024;
   o->blocks_per_cu = 2;
   o->splitter = GGMS_SPLIT_LINEAR;
-  o->learned_pieces = 256;
+  o->learned_pieces = 64;
   o->max_passes = 64;
   o->seed = 0x5EEDull;
   o->stream = nullptr;

@@ -397,7 +397,7 @@ template <class K> struct LutGeom {
   static constexpr uint32_t L1CELLS = 1u << L1BITS;
   static constexpr uint32_t S1 = KB - L1BITS;
   static constexpr uint32_t S2 = S1 - 6;
-  static constexpr uint32_t L2TABLES = sizeof(K) == 4 ? 16u : 32u;  // 4 KB (u32) / 16 KB (u64) of level 2
+  static constexpr uint32_t L2TABLES = 32u;  // 8 KB (u32) / 16 KB (u64) of level 2
   static constexpr K FLAG = (K)1 << (KB - 1);
   static constexpr K TABLE = (K)1 << (KB - 2);
   static constexpr K M1 = ((K)1 << S1) - 1;

@@ -58,7 +58,7 @@ typedef struct {
   uint32_t block_threads; /* workgroup size of the streaming kernels (default 1024) */
   uint32_t blocks_per_cu; /* streaming workgroups per CU (default 2) */
   uint32_t splitter;      /* ggms_splitter (default LINEAR) */
-  uint32_t learned_pieces;/* ranges produced by the learned / sketch splitter (default 256) */
+  uint32_t learned_pieces;/* ranges produced by the learned / sketch splitter (default 64) */
   uint32_t max_passes;    /* safety cap on bucketing passes (default 64) */
   uint64_t seed;          /* sampling seed (default 0x5EED) */
   void* stream;           /* hipStream_t to run on; NULL = the null stream */
