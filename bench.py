@@ -43,6 +43,41 @@ def measured_traffic(kernel):
     return None, None
 
 
+def bench_other_configs(ctx, args):
+    """The remaining benchmark inputs, timed like the headline (linear splitter, iterativeSMOS):
+    2^26 adversarial inputs with 8192 evenly spaced or clustered ranks, a 2^28 fp32 Gaussian
+    mixture with 8192 random ranks, and 2^27 fp64 N(0,1) with 1001 random ranks.  Exactness at
+    these sizes is checked by tests/test_gpu_configs.py."""
+    import torch
+    import ggms
+    cases = [("uint32 U[0,1023]", "uint32", "uniform", 26, 1024.0, 0.0, "even", 8192),
+             ("uint32 U[0,1023]", "uint32", "uniform", 26, 1024.0, 0.0, "window", 8192),
+             ("fp32 |N(0,1)|", "float32", "half_normal", 26, 0.0, 0.0, "even", 8192),
+             ("fp32 99% 1.0f", "float32", "near_constant", 26, 1.0, 0.01, "even", 8192),
+             ("fp32 99% 1.0f", "float32", "near_constant", 26, 1.0, 0.01, "window", 8192),
+             ("fp32 mixture", "float32", "mixture", 28, 0.0, 0.0, "uniform_random", 8192),
+             ("fp64 N(0,1)", "float64", "normal", 27, 0.0, 0.0, "uniform_random", 1001)]
+    out = []
+    for label, dt, dist, lg, p0, p1, kd, nk in cases:
+        n = 1 << lg
+        x = ggms.generate(dt, dist, n, args.seed + lg, p0, p1)
+        ks = ggms.generate_ks(kd, nk, n, args.seed + nk)
+        o = ggms.default_opts(splitter="linear")
+        ctx.select(x, ks, "iterative", o)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        reps = 5
+        for _ in range(reps):
+            ctx.select(x, ks, "iterative", o)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) / reps * 1e3
+        st = ctx.stats()
+        out.append({"workload": f"{label}, N=2^{lg}, {nk} ks {kd}", "ms_per_step": round(ms, 4),
+                    "elements_per_s": round(n / (ms * 1e-3), 1), "passes": st["passes"]})
+        del x
+    return out
+
+
 def bench_batched(args):
     """Batched workload: 4096 problems, sizes log-uniform in [2^10, 2^22], each
     uniform / normal / lognormal(0,1) fp32 (seeded per problem), 1..512 random
@@ -103,6 +138,7 @@ def main():
                     help="also run one plain device copy of the input (known bytes: 4N read, 4N written) so "
                          "hardware byte counters can be scaled")
     ap.add_argument("--no-forms", action="store_true", help="skip the learned and sketch forms")
+    ap.add_argument("--no-extra", action="store_true", help="skip the adversarial / mixture / fp64 workloads")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: the multi-GPU form (one histogram all-reduce "
@@ -227,6 +263,8 @@ def main():
 
     if not args.no_batch:
         result["batched"] = bench_batched(args)
+    if not args.no_extra:
+        result["other_configs"] = bench_other_configs(ctx, args)
 
     if not args.no_check:
         import oracle

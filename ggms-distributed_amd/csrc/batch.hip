@@ -78,7 +78,7 @@ template <class K> struct BatchLds {
   uint32_t* Ko;     // sorted ks: output index
   K* starts;        // range starts of the first pass
   uint32_t* req;    // requested buckets per range
-  MapParam<K>* prm; // map parameters per range
+  MapParam<K>* prm; // map parameters per range (the first-pass map's parameter arrays)
   uint32_t* rbase;  // first bucket per range
   uint32_t* cursor; // per child item: next write position
   uint32_t* runb;   // per k: its bucket
@@ -375,6 +375,8 @@ __global__ void __launch_bounds__(1024) k_batch(BatchArgs a) {
     }
     __syncthreads();
     const uint32_t R = s_n;
+    Pass0Map<K> mp;
+    mp.set_params(reinterpret_cast<unsigned char*>(L.prm), R);
     {
       uint32_t sum = 0;
       uint32_t per = (R + blockDim.x - 1) / blockDim.x;
@@ -396,7 +398,7 @@ __global__ void __launch_bounds__(1024) k_batch(BatchArgs a) {
         make_map<K>(rr, L.req[r]);
         rr.base = o;
         L.rbase[r] = o;
-        L.prm[r] = to_param<K>(rr);
+        mp.store_param(r, rr);
         o += rr.nb;
       }
       if (threadIdx.x == 0) s_n = tot;
@@ -405,9 +407,7 @@ __global__ void __launch_bounds__(1024) k_batch(BatchArgs a) {
     const uint32_t B0 = s_n;
     build_lut<K>(L.starts, R, L.A);  // cell table in LDS (sample no longer needed)
     __syncthreads();
-    Pass0Map<K> mp;
     mp.lut = L.A;
-    mp.prm = L.prm;
     for (uint32_t i = threadIdx.x; i < B0; i += blockDim.x) L.H[i] = 0;
     __syncthreads();
     constexpr int NV = Vec16<T>::N;

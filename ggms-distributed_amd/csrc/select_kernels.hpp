@@ -426,19 +426,54 @@ template <class K> __device__ __forceinline__ uint32_t param_bucket(const MapPar
   return q.base + h + id;
 }
 
+// The per-range map parameters are stored as separate arrays (lo, mult,
+// base, and sh for 64-bit keys): lanes of a wave that read different ranges
+// then hit different LDS banks (range r sits in bank r mod 32 of each array),
+// where one packed record per range made ranges r and r + 8 collide.
 template <class K> struct Pass0Map {
   using G = LutGeom<K>;
   K* lut;  // level 1 followed by the level-2 tables
-  MapParam<K>* prm;
+  K* plo;
+  uint32_t* pmult;
+  uint32_t* pbase;
+  uint32_t* psh;
 
-  static __host__ __device__ size_t bytes(uint32_t R) { return G::bytes() + (size_t)R * sizeof(MapParam<K>); }
-  __device__ void carve(unsigned char* p) {
+  // LDS bytes of the parameter arrays for R ranges (<= R packed records)
+  static __host__ __device__ size_t param_bytes(uint32_t R) {
+    return (((size_t)R * (sizeof(K) + 12)) + 15) / 16 * 16;
+  }
+  static __host__ __device__ size_t bytes(uint32_t R) { return G::bytes() + param_bytes(R); }
+  __device__ void set_params(unsigned char* p, uint32_t R) {
+    plo = reinterpret_cast<K*>(p);
+    pmult = reinterpret_cast<uint32_t*>(p + (size_t)R * sizeof(K));
+    pbase = pmult + R;
+    psh = pbase + R;
+  }
+  __device__ void carve(unsigned char* p, uint32_t R) {
     lut = reinterpret_cast<K*>(p);
-    prm = reinterpret_cast<MapParam<K>*>(p + G::bytes());
+    set_params(p + G::bytes(), R);
+  }
+  __device__ __forceinline__ void store_param(uint32_t r, const Range<K>& g) {
+    plo[r] = g.lo;
+    pmult[r] = g.mult;
+    pbase[r] = g.base;
+    psh[r] = g.sh;
+  }
+  __device__ void load_params(const Range<K>* __restrict__ g, uint32_t R) {
+    for (uint32_t i = threadIdx.x; i < R; i += blockDim.x) store_param(i, g[i]);
   }
   __device__ void load(const K* __restrict__ g_lut, const Range<K>* __restrict__ g, uint32_t R) {
     for (uint32_t i = threadIdx.x; i < G::L1CELLS + G::L2TABLES * 64; i += blockDim.x) lut[i] = g_lut[i];
-    for (uint32_t i = threadIdx.x; i < R; i += blockDim.x) prm[i] = to_param<K>(g[i]);
+    load_params(g, R);
+  }
+  // bucket of key in range r (branch-free, as param_bucket)
+  __device__ __forceinline__ uint32_t range_bucket(uint32_t r, K key) const {
+    K d = key - plo[r];
+    uint32_t t = sizeof(K) == 4 ? (uint32_t)d : (uint32_t)(d >> psh[r]);
+    uint32_t m = pmult[r];
+    uint32_t h = __umulhi(t, m);
+    uint32_t id = m == 0 ? t : 0u;
+    return pbase[r] + h + id;
   }
   // Range of a key in a level-1 leaf entry: r0 (the bits above S1; the flag
   // bit is 0 for a leaf) plus one when the key's in-cell offset passes the
@@ -463,7 +498,7 @@ template <class K> struct Pass0Map {
           uint32_t lo = (uint32_t)(e & 0x7FF), hi = (uint32_t)((e >> 11) & 0x7FF);
           while (lo < hi) {
             uint32_t mid = (lo + hi + 1) >> 1;
-            if (prm[mid].lo <= key) lo = mid; else hi = mid - 1;
+            if (plo[mid] <= key) lo = mid; else hi = mid - 1;
           }
           r = lo;
         }
@@ -471,7 +506,7 @@ template <class K> struct Pass0Map {
     }
     return r;
   }
-  __device__ __forceinline__ uint32_t bucket(K key) const { return param_bucket<K>(prm[range_of(key)], key); }
+  __device__ __forceinline__ uint32_t bucket(K key) const { return range_bucket(range_of(key), key); }
 
   // Buckets of N keys: the N level-1 reads are issued together, one ballot
   // covers the rare non-leaf cells of all N, then the N parameter reads.
@@ -499,7 +534,7 @@ template <class K> struct Pass0Map {
             uint32_t lo = (uint32_t)(ej & 0x7FF), hi = (uint32_t)((ej >> 11) & 0x7FF);
             while (lo < hi) {
               uint32_t mid = (lo + hi + 1) >> 1;
-              if (prm[mid].lo <= key[j]) lo = mid; else hi = mid - 1;
+              if (plo[mid] <= key[j]) lo = mid; else hi = mid - 1;
             }
             r[j] = lo;
           }
@@ -507,7 +542,7 @@ template <class K> struct Pass0Map {
       }
     }
 #pragma unroll
-    for (int j = 0; j < N; ++j) b[j] = param_bucket<K>(prm[r[j]], key[j]);
+    for (int j = 0; j < N; ++j) b[j] = range_bucket(r[j], key[j]);
   }
 };
 
@@ -616,8 +651,8 @@ __global__ void __launch_bounds__(1024) k_count0(const T* __restrict__ x, const 
   __shared__ __align__(16) K s_lut[LutGeom<K>::kEntries];
   Pass0Map<K> mp;
   mp.lut = s_lut;
-  mp.prm = reinterpret_cast<MapParam<K>*>(smem);
-  uint32_t* hist = reinterpret_cast<uint32_t*>(smem + (size_t)ctl->R * sizeof(MapParam<K>));
+  mp.set_params(smem, ctl->R);
+  uint32_t* hist = reinterpret_cast<uint32_t*>(smem + Pass0Map<K>::param_bytes(ctl->R));
   for (uint32_t i = threadIdx.x; i < B; i += blockDim.x) hist[i] = 0;
   mp.load(lut, ranges, ctl->R);
   __syncthreads();
@@ -1082,8 +1117,8 @@ __global__ void __launch_bounds__(1024) k_compact0(const T* __restrict__ x, cons
   __shared__ __align__(16) K s_lut[LutGeom<K>::kEntries];  // static: folded cell addresses
   Pass0Map<K> mp;
   mp.lut = s_lut;
-  mp.prm = reinterpret_cast<MapParam<K>*>(smem);
-  uint16_t* slot = reinterpret_cast<uint16_t*>(smem + (size_t)ctl->R * sizeof(MapParam<K>));
+  mp.set_params(smem, ctl->R);
+  uint16_t* slot = reinterpret_cast<uint16_t*>(smem + Pass0Map<K>::param_bytes(ctl->R));
   for (uint32_t i = threadIdx.x; i < B; i += blockDim.x) slot[i] = slot_of_bucket[i];
   mp.load(lut, ranges, ctl->R);
   const uint32_t* rp = rows + (size_t)blockIdx.x * B;
@@ -1175,7 +1210,7 @@ __device__ __forceinline__ void pass1_map_init(Pass1Map<K>& pm, unsigned char* p
                                                const K* lut, const Range<K>* rng0, const uint16_t* slot0_g,
                                                const MapParam<K>* p1_g, uint32_t R1, bool p1_in_lds) {
   const uint32_t B0 = ctl0->B, R0 = ctl0->R;
-  pm.m0.carve(p);  // first-pass cell table first (p is LDS offset 0 in the pass-1 kernels)
+  pm.m0.carve(p, R0);  // first-pass cell table first (p is LDS offset 0 in the pass-1 kernels)
   pm.m0.load(lut, rng0, R0);
   uint16_t* s0 = reinterpret_cast<uint16_t*>(p + Pass0Map<K>::bytes(R0));
   for (uint32_t i = threadIdx.x; i < B0; i += blockDim.x) s0[i] = slot0_g[i];

@@ -61,9 +61,9 @@ int run(uint32_t mode, uint32_t S, uint32_t P, uint32_t B, int kind, uint64_t se
   if (base != ctl.B) bad++;
   // reference map by binary search vs LUT-based map
   std::vector<uint32_t> starts(R); for (uint32_t r = 0; r < R; ++r) starts[r] = ranges[r].lo;
-  std::vector<MapParam<uint32_t>> prm(R);
-  for (uint32_t r = 0; r < R; ++r) { prm[r].lo = ranges[r].lo; prm[r].base = ranges[r].base; prm[r].mult = ranges[r].mult; prm[r].sh = ranges[r].sh; }
-  Pass0Map<uint32_t> mp; mp.lut = lut.data(); mp.prm = prm.data();
+  std::vector<unsigned char> prm(Pass0Map<uint32_t>::param_bytes(R) + 16);
+  Pass0Map<uint32_t> mp; mp.lut = lut.data(); mp.set_params(prm.data(), R);
+  for (uint32_t r = 0; r < R; ++r) mp.store_param(r, ranges[r]);
   for (int t = 0; t < 200000; ++t) {
     uint32_t key = (uint32_t)rng();
     if (t % 2) key = KeyTraits<float>::enc(x[rng() % n]);
