@@ -41,8 +41,9 @@ hipError_t& last_hip_error() {
 
 static inline size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 
+constexpr int kMaxBuffers = 48;  // capacity of Layout::off; every workspace buffer needs an entry
 struct Layout {
-  size_t off[32];
+  size_t off[kMaxBuffers];
   size_t total;
 };
 
@@ -214,6 +215,7 @@ enum Buf {
   bSmallStart, bSmallLen, bSmallKStart, bSmallKFill, bWgFirst, bOut, bLut, bSmallKlo, bSmallKhi, bParams1, bSid,
   bCount
 };
+static_assert(bCount <= kMaxBuffers, "workspace layout table too small");
 
 template <class T>
 static Layout make_layout(uint64_t n, uint32_t nk, const Config& c) {
@@ -425,6 +427,15 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
       uint32_t* wgf_next = (uint32_t*)P(bWgFirst) + (size_t)((pass + 1) & 1) * c.G_max;
       const uint32_t NT = c.NT;
       uint16_t* sid = (uint16_t*)P(bSid);  // first-pass slot of each survivor (pass 0 writes, pass 1 reads)
+      // pass-1 range parameters live in LDS unless that would push either
+      // pass-1 kernel past 112 KB (then they are read from global memory)
+      uint32_t p1_lds = 0;
+      if (pass == 1) {
+        size_t cnt = Pass0Map<K>::param_bytes(R_bound) + (size_t)B_bound * 4;
+        size_t cmp = Pass0Map<K>::param_bytes(R_bound) + compact_lds_bytes(B_bound, U_slot_bound + 4);
+        p1_lds = std::max(cnt, cmp) <= 112 * 1024 ? 1u : 0u;
+      }
+      const size_t p1_bytes = p1_lds ? Pass0Map<K>::param_bytes(R_bound) : 0;
       // count
       if (pass == 0) {
         size_t lds = count_lds0<T>(c);
@@ -434,11 +445,11 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
         k_count0<T><<<G, NT, lds, st>>>(d_vec, rc, (K*)P(bLut), cp, rows);
         prof.end();
       } else if (pass == 1) {
-        size_t lds = Pass0Map<K>::param_bytes(R_bound) + (size_t)B_bound * 4;
+        size_t lds = p1_bytes + (size_t)B_bound * 4;
         LDS_CHECK(lds);
         set_lds_attr((const void*)k_count1<K>, lds);
         prof.begin(GGMS_KERNEL_COUNTS, (int)pass);
-        k_count1<K><<<G, NT, lds, st>>>(src, sid, (MapParam<K>*)P(bParams1), cp, rows);
+        k_count1<K><<<G, NT, lds, st>>>(src, sid, (MapParam<K>*)P(bParams1), p1_lds, cp, rows);
         prof.end();
       } else {
         size_t lds = (size_t)B_bound * 4;
@@ -508,12 +519,12 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
                                           dst, sid);
         prof.end();
       } else if (pass == 1) {
-        size_t lds = Pass0Map<K>::param_bytes(R_bound) + compact_lds_bytes(B_bound, U_slot_bound + 4);
+        size_t lds = p1_bytes + compact_lds_bytes(B_bound, U_slot_bound + 4);
         LDS_CHECK(lds);
         set_lds_attr((const void*)k_compact1<K>, lds);
         prof.begin(GGMS_KERNEL_COMPACTS, (int)pass);
-        k_compact1<K><<<G, NT, lds, st>>>(src, sid, (MapParam<K>*)P(bParams1), cp, rows, pl.slot_of_bucket,
-                                          pl.slot_dst, pl.slot_bucket, dst);
+        k_compact1<K><<<G, NT, lds, st>>>(src, sid, (MapParam<K>*)P(bParams1), p1_lds, cp, rows,
+                                          pl.slot_of_bucket, pl.slot_dst, pl.slot_bucket, dst);
         prof.end();
       } else {
         size_t lds = compact_lds_bytes(B_bound, U_slot_bound + 4);

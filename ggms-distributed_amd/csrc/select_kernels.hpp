@@ -1239,29 +1239,40 @@ __device__ __forceinline__ void stream_tile_sid(const K* __restrict__ x, const u
   for (uint32_t i = tail0 + threadIdx.x; i < t1; i += nt) f1(x[i], (uint32_t)sid[i]);
 }
 
-template <class K>
-__device__ __forceinline__ void pass1_params_init(Pass0Map<K>& pm, unsigned char* p, const MapParam<K>* __restrict__ g,
-                                                  uint32_t R) {
-  pm.set_params(p, R);
-  for (uint32_t i = threadIdx.x; i < R; i += blockDim.x) {
-    const MapParam<K> q = g[i];
-    pm.plo[i] = q.lo;
-    pm.pmult[i] = q.mult;
-    pm.pbase[i] = q.base;
-    pm.psh[i] = q.sh;
+// Pass-1 map: the ranges' parameters as per-field LDS arrays when they fit
+// (in_lds), else read from the packed global table.  Returns the LDS bytes used.
+template <class K> struct Pass1Params {
+  Pass0Map<K> pm;
+  const MapParam<K>* g;
+  bool in_lds;
+  __device__ __forceinline__ size_t init(unsigned char* p, const MapParam<K>* __restrict__ g_, uint32_t R, bool lds) {
+    g = g_;
+    in_lds = lds;
+    if (!lds) return 0;
+    pm.set_params(p, R);
+    for (uint32_t i = threadIdx.x; i < R; i += blockDim.x) {
+      const MapParam<K> q = g_[i];
+      pm.plo[i] = q.lo;
+      pm.pmult[i] = q.mult;
+      pm.pbase[i] = q.base;
+      pm.psh[i] = q.sh;
+    }
+    return Pass0Map<K>::param_bytes(R);
   }
-}
+  __device__ __forceinline__ uint32_t bucket(uint32_t s, K key) const {
+    return in_lds ? pm.range_bucket(s, key) : param_bucket<K>(g[s], key);
+  }
+};
 
 template <class K>
 __global__ void __launch_bounds__(1024) k_count1(const K* __restrict__ buf, const uint16_t* __restrict__ sid,
-                                                 const MapParam<K>* __restrict__ p1, const PassCtl* __restrict__ ctl,
-                                                 uint32_t* __restrict__ rows) {
+                                                 const MapParam<K>* __restrict__ p1, uint32_t p1_lds,
+                                                 const PassCtl* __restrict__ ctl, uint32_t* __restrict__ rows) {
   extern __shared__ __align__(16) unsigned char smem[];
   const uint32_t B = ctl->B, n = ctl->n, tile = ctl->tile, R = ctl->R;
   if (blockIdx.x >= ctl->G) return;
-  Pass0Map<K> pm;
-  pass1_params_init<K>(pm, smem, p1, R);
-  uint32_t* hist = reinterpret_cast<uint32_t*>(smem + Pass0Map<K>::param_bytes(R));
+  Pass1Params<K> pm;
+  uint32_t* hist = reinterpret_cast<uint32_t*>(smem + pm.init(smem, p1, R, p1_lds != 0));
   for (uint32_t i = threadIdx.x; i < B; i += blockDim.x) hist[i] = 0;
   __syncthreads();
   uint32_t t0 = blockIdx.x * tile;
@@ -1272,10 +1283,10 @@ __global__ void __launch_bounds__(1024) k_count1(const K* __restrict__ buf, cons
       [&](const K (&k)[N], const uint32_t (&s)[N]) {
         uint32_t b[N];
 #pragma unroll
-        for (int j = 0; j < N; ++j) b[j] = pm.range_bucket(s[j], k[j]);
+        for (int j = 0; j < N; ++j) b[j] = pm.bucket(s[j], k[j]);
         lds_count_n<N>(hist, b);
       },
-      [&](K key, uint32_t s) { lds_count(hist, pm.range_bucket(s, key)); });
+      [&](K key, uint32_t s) { lds_count(hist, pm.bucket(s, key)); });
   __syncthreads();
   uint32_t* row = rows + (size_t)blockIdx.x * B;
   for (uint32_t i = threadIdx.x; i < B; i += blockDim.x) row[i] = hist[i];
@@ -1283,19 +1294,18 @@ __global__ void __launch_bounds__(1024) k_count1(const K* __restrict__ buf, cons
 
 template <class K>
 __global__ void __launch_bounds__(1024) k_compact1(const K* __restrict__ buf, const uint16_t* __restrict__ sid,
-                                                   const MapParam<K>* __restrict__ p1, const PassCtl* __restrict__ ctl,
-                                                   const uint32_t* __restrict__ rows,
+                                                   const MapParam<K>* __restrict__ p1, uint32_t p1_lds,
+                                                   const PassCtl* __restrict__ ctl, const uint32_t* __restrict__ rows,
                                                    const uint16_t* __restrict__ slot_of_bucket,
                                                    const uint32_t* __restrict__ slot_dst,
                                                    const uint32_t* __restrict__ slot_bucket, K* __restrict__ out) {
   extern __shared__ __align__(16) unsigned char smem[];
   const uint32_t B = ctl->B, n = ctl->n, tile = ctl->tile, U_slots = ctl->U_slots, R = ctl->R;
   if (blockIdx.x >= ctl->G || U_slots == 0) return;
-  Pass0Map<K> pm;
-  pass1_params_init<K>(pm, smem, p1, R);
+  Pass1Params<K> pm;
+  const size_t pb = pm.init(smem, p1, R, p1_lds != 0);
   CompactTables ct;
-  compact_init(ct, smem + Pass0Map<K>::param_bytes(R), B, U_slots, slot_of_bucket, slot_dst, slot_bucket,
-               rows + (size_t)blockIdx.x * B);
+  compact_init(ct, smem + pb, B, U_slots, slot_of_bucket, slot_dst, slot_bucket, rows + (size_t)blockIdx.x * B);
   __syncthreads();
   uint32_t t0 = blockIdx.x * tile;
   uint32_t t1 = t0 + tile < n ? t0 + tile : n;
@@ -1311,11 +1321,11 @@ __global__ void __launch_bounds__(1024) k_compact1(const K* __restrict__ buf, co
       [&](const K (&k)[N], const uint32_t (&s)[N]) {
         uint32_t b[N];
 #pragma unroll
-        for (int j = 0; j < N; ++j) b[j] = pm.range_bucket(s[j], k[j]);
+        for (int j = 0; j < N; ++j) b[j] = pm.bucket(s[j], k[j]);
 #pragma unroll
         for (int j = 0; j < N; ++j) put(k[j], b[j]);
       },
-      [&](K key, uint32_t s) { put(key, pm.range_bucket(s, key)); });
+      [&](K key, uint32_t s) { put(key, pm.bucket(s, key)); });
 }
 
 template <class K>
