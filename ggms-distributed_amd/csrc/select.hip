@@ -212,8 +212,7 @@ struct Prof {
 enum Buf {
   bBufA, bBufB, bRows, bTotals, bRanges0, bRanges1, bRangesFirst, bCtl, bKs, bKRank, bKState, bKAux, bKList,
   bSlotOfBucket, bSlotDst, bSlotBucket, bActBucket, bActCls, bActDst, bActSlot, bActKlo, bActKhi,
-  bSmallStart, bSmallLen, bSmallKStart, bSmallKFill, bWgFirst, bOut, bLut, bSmallKlo, bSmallKhi, bParams1, bSid,
-  bCount
+  bSmallStart, bSmallLen, bSmallKStart, bSmallKFill, bWgFirst, bOut, bLut, bSmallKlo, bSmallKhi, bParams1, bCount
 };
 static_assert(bCount <= kMaxBuffers, "workspace layout table too small");
 
@@ -255,7 +254,6 @@ static Layout make_layout(uint64_t n, uint32_t nk, const Config& c) {
   sz[bOut] = nkp * 8;
   sz[bLut] = LutGeom<K>::bytes();
   sz[bParams1] = Bm * sizeof(MapParam<K>);
-  sz[bSid] = n <= c.fin_cap ? 0 : (size_t)n * 2;  // first-pass slot of every survivor
   sz[bSmallKlo] = Bm * sizeof(K);
   sz[bSmallKhi] = Bm * sizeof(K);
   Layout L{};
@@ -426,16 +424,14 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
       uint32_t* wgf_cur = (uint32_t*)P(bWgFirst) + (size_t)(pass & 1) * c.G_max;
       uint32_t* wgf_next = (uint32_t*)P(bWgFirst) + (size_t)((pass + 1) & 1) * c.G_max;
       const uint32_t NT = c.NT;
-      uint16_t* sid = (uint16_t*)P(bSid);  // first-pass slot of each survivor (pass 0 writes, pass 1 reads)
       // pass-1 range parameters live in LDS unless that would push either
       // pass-1 kernel past 112 KB (then they are read from global memory)
       uint32_t p1_lds = 0;
       if (pass == 1) {
-        size_t cnt = Pass0Map<K>::param_bytes(R_bound) + (size_t)B_bound * 4;
-        size_t cmp = Pass0Map<K>::param_bytes(R_bound) + compact_lds_bytes(B_bound, U_slot_bound + 4);
+        size_t cnt = ((size_t)B_bound * 4 + 15) / 16 * 16 + pass1_map_bytes<K>(B0, R0, R_bound);
+        size_t cmp = compact_lds_bytes(B_bound, U_slot_bound + 4) + pass1_map_bytes<K>(B0, R0, R_bound);
         p1_lds = std::max(cnt, cmp) <= 112 * 1024 ? 1u : 0u;
       }
-      const size_t p1_bytes = p1_lds ? Pass0Map<K>::param_bytes(R_bound) : 0;
       // count
       if (pass == 0) {
         size_t lds = count_lds0<T>(c);
@@ -445,11 +441,12 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
         k_count0<T><<<G, NT, lds, st>>>(d_vec, rc, (K*)P(bLut), cp, rows);
         prof.end();
       } else if (pass == 1) {
-        size_t lds = p1_bytes + (size_t)B_bound * 4;
+        size_t lds = ((size_t)B_bound * 4 + 15) / 16 * 16 + pass1_map_bytes<K>(B0, R0, p1_lds ? R_bound : 0);
         LDS_CHECK(lds);
         set_lds_attr((const void*)k_count1<K>, lds);
         prof.begin(GGMS_KERNEL_COUNTS, (int)pass);
-        k_count1<K><<<G, NT, lds, st>>>(src, sid, (MapParam<K>*)P(bParams1), p1_lds, cp, rows);
+        k_count1<K><<<G, NT, lds, st>>>(src, ctl, (K*)P(bLut), rng_first, slot_tab[0],
+                                        (MapParam<K>*)P(bParams1), p1_lds, cp, rows);
         prof.end();
       } else {
         size_t lds = (size_t)B_bound * 4;
@@ -516,14 +513,15 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
         set_lds_attr((const void*)k_compact0<T>, lds);
         prof.begin(GGMS_KERNEL_COMPACT0, (int)pass);
         k_compact0<T><<<G, NT, lds, st>>>(d_vec, rc, (K*)P(bLut), cp, rows, pl.slot_of_bucket, pl.slot_bucket,
-                                          dst, sid);
+                                          dst);
         prof.end();
       } else if (pass == 1) {
-        size_t lds = p1_bytes + compact_lds_bytes(B_bound, U_slot_bound + 4);
+        size_t lds = compact_lds_bytes(B_bound, U_slot_bound + 4) + pass1_map_bytes<K>(B0, R0, p1_lds ? R_bound : 0);
         LDS_CHECK(lds);
         set_lds_attr((const void*)k_compact1<K>, lds);
         prof.begin(GGMS_KERNEL_COMPACTS, (int)pass);
-        k_compact1<K><<<G, NT, lds, st>>>(src, sid, (MapParam<K>*)P(bParams1), p1_lds, cp, rows,
+        k_compact1<K><<<G, NT, lds, st>>>(src, ctl, (K*)P(bLut), rng_first, slot_tab[0],
+                                          (MapParam<K>*)P(bParams1), p1_lds, cp, rows,
                                           pl.slot_of_bucket, pl.slot_dst, pl.slot_bucket, dst);
         prof.end();
       } else {

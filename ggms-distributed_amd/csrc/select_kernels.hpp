@@ -1107,8 +1107,7 @@ __global__ void __launch_bounds__(1024) k_compact0(const T* __restrict__ x, cons
                                                    const PassCtl* __restrict__ ctl, const uint32_t* __restrict__ rows,
                                                    const uint16_t* __restrict__ slot_of_bucket,
                                                    const uint32_t* __restrict__ slot_bucket,
-                                                   typename KeyTraits<T>::K* __restrict__ out,
-                                                   uint16_t* __restrict__ sid_out) {
+                                                   typename KeyTraits<T>::K* __restrict__ out) {
   using K = typename KeyTraits<T>::K;
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ uint32_t cursor;
@@ -1143,13 +1142,11 @@ __global__ void __launch_bounds__(1024) k_compact0(const T* __restrict__ x, cons
         uint32_t b[N];
         mp.template buckets<N>(k, b);
         bool act[N];
-        uint32_t sl[N];
         uint64_t m[N];
         uint32_t tot = 0;
 #pragma unroll
         for (int j = 0; j < N; ++j) {
-          sl[j] = slot[b[j]];
-          act[j] = sl[j] != kNone16;
+          act[j] = slot[b[j]] != kNone16;
           m[j] = wave_ballot(act[j]);
           tot += (uint32_t)__builtin_popcountll(m[j]);
         }
@@ -1160,172 +1157,159 @@ __global__ void __launch_bounds__(1024) k_compact0(const T* __restrict__ x, cons
           base = readlane_u32(base, first);
 #pragma unroll
           for (int j = 0; j < N; ++j) {
-            if (act[j]) {
-              const uint32_t p = base + mask_below(m[j]);
-              out[p] = k[j];
-              sid_out[p] = (uint16_t)sl[j];
-            }
+            if (act[j]) out[base + mask_below(m[j])] = k[j];
             base += (uint32_t)__builtin_popcountll(m[j]);
           }
         }
       },
       [&](T v) {
         K key = KeyTraits<T>::enc(v);
-        const uint32_t s1 = slot[mp.bucket(key)];
-        bool act = s1 != kNone16;
+        bool act = slot[mp.bucket(key)] != kNone16;
         uint64_t m = wave_ballot(act);
         if (m) {
           uint32_t first = (uint32_t)__builtin_ctzll(m);
           uint32_t base = 0;
           if (lane_id() == first) base = atomicAdd(&cursor, (uint32_t)__builtin_popcountll(m));
           base = readlane_u32(base, first);
-          if (act) {
-            const uint32_t p = base + mask_below(m);
-            out[p] = key;
-            sid_out[p] = (uint16_t)s1;
-          }
+          if (act) out[base + mask_below(m)] = key;
         }
       });
 }
 
-// Pass 1 reads the append-layout buffer together with the first-pass slot of
-// every survivor (written beside it by the first-pass compaction): the slot is
-// the survivor's pass-1 range, so an element's pass-1 bucket is one parameter
-// lookup.  The pass-1 ranges' parameters are staged in LDS as per-field arrays.
-template <class K, int N>
-__device__ __forceinline__ void load_sids(const uint16_t* __restrict__ sid, uint32_t v, uint32_t (&s)[N]) {
-  if constexpr (N == 4) {
-    const uint2 w = reinterpret_cast<const uint2*>(sid)[v];
-    s[0] = w.x & 0xFFFFu; s[1] = w.x >> 16; s[2] = w.y & 0xFFFFu; s[3] = w.y >> 16;
-  } else {
-    const uint32_t w = reinterpret_cast<const uint32_t*>(sid)[v];
-    s[0] = w & 0xFFFFu; s[1] = w >> 16;
+// Pass 1 reads the append-layout buffer: an element's range is the slot of its
+// first-pass bucket (recomputed from the key with the first-pass map), and the
+// range's own map gives the pass-1 bucket.  The pass-1 ranges' parameters are
+// staged in LDS, indexed by slot.
+template <class K> struct Pass1Map {
+  Pass0Map<K> m0;
+  const uint16_t* slot0;               // LDS: first-pass bucket -> slot (= pass-1 range)
+  const MapParam<K>* p1;               // pass-1 range parameters by slot, global memory
+  Pass0Map<K> p1s;                     // ... or as per-field LDS arrays (p1_lds)
+  bool p1_lds;
+  __device__ __forceinline__ uint32_t range_bucket(uint32_t s0, K key) const {
+    return p1_lds ? p1s.range_bucket(s0, key) : param_bucket<K>(p1[s0], key);
   }
-}
-
-// Streams keys [t0, t1) (t0 a multiple of the vector width) with their slots:
-// fv(keys[N], slots[N]) per 16-byte load, f1(key, slot) for the tail.
-template <class K, class FV, class F1>
-__device__ __forceinline__ void stream_tile_sid(const K* __restrict__ x, const uint16_t* __restrict__ sid, uint32_t t0,
-                                                uint32_t t1, FV&& fv, F1&& f1) {
-  constexpr int N = Vec16<K>::N;
-  const uint32_t nt = blockDim.x;
-  const uint4* xv = reinterpret_cast<const uint4*>(x);
-  uint32_t v1 = t1 / N;
-  uint32_t v = t0 / N + threadIdx.x;
-  for (; v + 3 * nt < v1; v += 4 * nt) {
-    uint4 q[4];
-    uint32_t sl[4][N];
+  __device__ __forceinline__ uint32_t bucket(K key) const { return range_bucket(slot0[m0.bucket(key)], key); }
+  template <int N> __device__ __forceinline__ void buckets(const K (&key)[N], uint32_t (&b)[N]) const {
+    uint32_t b0[N];
+    m0.template buckets<N>(key, b0);
+    uint32_t s0[N];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) q[u] = xv[v + u * nt];
+    for (int j = 0; j < N; ++j) s0[j] = slot0[b0[j]];
+    if (p1_lds) {
 #pragma unroll
-    for (int u = 0; u < 4; ++u) load_sids<K, N>(sid, v + u * nt, sl[u]);
+      for (int j = 0; j < N; ++j) b[j] = p1s.range_bucket(s0[j], key[j]);
+    } else {
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      K e[N];
-      Vec16<K>::unpack(q[u], e);
-      fv(e, sl[u]);
+      for (int j = 0; j < N; ++j) b[j] = param_bucket<K>(p1[s0[j]], key[j]);
     }
-  }
-  for (; v < v1; v += nt) {
-    uint4 q0 = xv[v];
-    uint32_t sl[N];
-    load_sids<K, N>(sid, v, sl);
-    K e[N];
-    Vec16<K>::unpack(q0, e);
-    fv(e, sl);
-  }
-  uint32_t tail0 = (v1 * N > t0) ? v1 * N : t0;
-  for (uint32_t i = tail0 + threadIdx.x; i < t1; i += nt) f1(x[i], (uint32_t)sid[i]);
-}
-
-// Pass-1 map: the ranges' parameters as per-field LDS arrays when they fit
-// (in_lds), else read from the packed global table.  Returns the LDS bytes used.
-template <class K> struct Pass1Params {
-  Pass0Map<K> pm;
-  const MapParam<K>* g;
-  bool in_lds;
-  __device__ __forceinline__ size_t init(unsigned char* p, const MapParam<K>* __restrict__ g_, uint32_t R, bool lds) {
-    g = g_;
-    in_lds = lds;
-    if (!lds) return 0;
-    pm.set_params(p, R);
-    for (uint32_t i = threadIdx.x; i < R; i += blockDim.x) {
-      const MapParam<K> q = g_[i];
-      pm.plo[i] = q.lo;
-      pm.pmult[i] = q.mult;
-      pm.pbase[i] = q.base;
-      pm.psh[i] = q.sh;
-    }
-    return Pass0Map<K>::param_bytes(R);
-  }
-  __device__ __forceinline__ uint32_t bucket(uint32_t s, K key) const {
-    return in_lds ? pm.range_bucket(s, key) : param_bucket<K>(g[s], key);
   }
 };
 
+// LDS bytes of the pass-1 map; R1 = 0 when the parameters stay in global memory.
 template <class K>
-__global__ void __launch_bounds__(1024) k_count1(const K* __restrict__ buf, const uint16_t* __restrict__ sid,
-                                                 const MapParam<K>* __restrict__ p1, uint32_t p1_lds,
-                                                 const PassCtl* __restrict__ ctl, uint32_t* __restrict__ rows) {
+__host__ __device__ __forceinline__ size_t pass1_map_bytes(uint32_t B0, uint32_t R0, uint32_t R1) {
+  return Pass0Map<K>::bytes(R0) + ((size_t)B0 * 2 + 15) / 16 * 16 + Pass0Map<K>::param_bytes(R1);
+}
+
+template <class K>
+__device__ __forceinline__ void pass1_map_init(Pass1Map<K>& pm, unsigned char* p, const PassCtl* ctl0,
+                                               const K* lut, const Range<K>* rng0, const uint16_t* slot0_g,
+                                               const MapParam<K>* p1_g, uint32_t R1, bool p1_in_lds) {
+  const uint32_t B0 = ctl0->B, R0 = ctl0->R;
+  pm.m0.carve(p, R0);  // first-pass cell table first (p is LDS offset 0 in the pass-1 kernels)
+  pm.m0.load(lut, rng0, R0);
+  uint16_t* s0 = reinterpret_cast<uint16_t*>(p + Pass0Map<K>::bytes(R0));
+  for (uint32_t i = threadIdx.x; i < B0; i += blockDim.x) s0[i] = slot0_g[i];
+  pm.slot0 = s0;
+  unsigned char* q = p + Pass0Map<K>::bytes(R0) + ((size_t)B0 * 2 + 15) / 16 * 16;
+  pm.p1 = p1_g;
+  pm.p1_lds = p1_in_lds;
+  if (p1_in_lds) {
+    pm.p1s.set_params(q, R1);
+    for (uint32_t i = threadIdx.x; i < R1; i += blockDim.x) {
+      const MapParam<K> g = p1_g[i];
+      pm.p1s.plo[i] = g.lo;
+      pm.p1s.pmult[i] = g.mult;
+      pm.p1s.pbase[i] = g.base;
+      pm.p1s.psh[i] = g.sh;
+    }
+  }
+}
+
+template <class K>
+__global__ void __launch_bounds__(1024) k_count1(const K* __restrict__ buf, const PassCtl* __restrict__ ctl0,
+                                                 const K* __restrict__ lut, const Range<K>* __restrict__ rng0,
+                                                 const uint16_t* __restrict__ slot0, const MapParam<K>* __restrict__ p1,
+                                                 uint32_t p1_in_lds, const PassCtl* __restrict__ ctl,
+                                                 uint32_t* __restrict__ rows) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const uint32_t B = ctl->B, n = ctl->n, tile = ctl->tile, R = ctl->R;
+  const uint32_t B = ctl->B, n = ctl->n, tile = ctl->tile;
   if (blockIdx.x >= ctl->G) return;
-  Pass1Params<K> pm;
-  uint32_t* hist = reinterpret_cast<uint32_t*>(smem + pm.init(smem, p1, R, p1_lds != 0));
+  Pass1Map<K> pm;
+  pass1_map_init<K>(pm, smem, ctl0, lut, rng0, slot0, p1, ctl->R, p1_in_lds != 0);
+  uint32_t* hist = reinterpret_cast<uint32_t*>(
+      smem + pass1_map_bytes<K>(ctl0->B, ctl0->R, p1_in_lds ? ctl->R : 0));
   for (uint32_t i = threadIdx.x; i < B; i += blockDim.x) hist[i] = 0;
   __syncthreads();
   uint32_t t0 = blockIdx.x * tile;
   uint32_t t1 = t0 + tile < n ? t0 + tile : n;
   constexpr int N = Vec16<K>::N;
-  stream_tile_sid<K>(
-      buf, sid, t0, t1,
-      [&](const K (&k)[N], const uint32_t (&s)[N]) {
+  stream_tile_vec<K>(
+      buf, t0, t1,
+      [&](const K (&k)[N]) {
         uint32_t b[N];
-#pragma unroll
-        for (int j = 0; j < N; ++j) b[j] = pm.bucket(s[j], k[j]);
+        pm.template buckets<N>(k, b);
         lds_count_n<N>(hist, b);
       },
-      [&](K key, uint32_t s) { lds_count(hist, pm.bucket(s, key)); });
+      [&](K key) { lds_count(hist, pm.bucket(key)); });
   __syncthreads();
   uint32_t* row = rows + (size_t)blockIdx.x * B;
   for (uint32_t i = threadIdx.x; i < B; i += blockDim.x) row[i] = hist[i];
 }
 
 template <class K>
-__global__ void __launch_bounds__(1024) k_compact1(const K* __restrict__ buf, const uint16_t* __restrict__ sid,
-                                                   const MapParam<K>* __restrict__ p1, uint32_t p1_lds,
-                                                   const PassCtl* __restrict__ ctl, const uint32_t* __restrict__ rows,
+__global__ void __launch_bounds__(1024) k_compact1(const K* __restrict__ buf, const PassCtl* __restrict__ ctl0,
+                                                   const K* __restrict__ lut, const Range<K>* __restrict__ rng0,
+                                                   const uint16_t* __restrict__ slot0, const MapParam<K>* __restrict__ p1,
+                                                   uint32_t p1_in_lds, const PassCtl* __restrict__ ctl,
+                                                   const uint32_t* __restrict__ rows,
                                                    const uint16_t* __restrict__ slot_of_bucket,
                                                    const uint32_t* __restrict__ slot_dst,
                                                    const uint32_t* __restrict__ slot_bucket, K* __restrict__ out) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const uint32_t B = ctl->B, n = ctl->n, tile = ctl->tile, U_slots = ctl->U_slots, R = ctl->R;
+  const uint32_t B = ctl->B, n = ctl->n, tile = ctl->tile, U_slots = ctl->U_slots;
   if (blockIdx.x >= ctl->G || U_slots == 0) return;
-  Pass1Params<K> pm;
-  const size_t pb = pm.init(smem, p1, R, p1_lds != 0);
+  Pass1Map<K> pm;
+  pass1_map_init<K>(pm, smem, ctl0, lut, rng0, slot0, p1, ctl->R, p1_in_lds != 0);
   CompactTables ct;
-  compact_init(ct, smem + pb, B, U_slots, slot_of_bucket, slot_dst, slot_bucket, rows + (size_t)blockIdx.x * B);
+  compact_init(ct, smem + pass1_map_bytes<K>(ctl0->B, ctl0->R, p1_in_lds ? ctl->R : 0), B, U_slots, slot_of_bucket,
+               slot_dst, slot_bucket, rows + (size_t)blockIdx.x * B);
   __syncthreads();
   uint32_t t0 = blockIdx.x * tile;
   uint32_t t1 = t0 + tile < n ? t0 + tile : n;
   constexpr int N = Vec16<K>::N;
-  auto put = [&](K key, uint32_t b) {
-    uint32_t s = ct.slot[b];
-    bool act = s != kNone16;
-    uint32_t p = lds_claim(ct.cursor, s, act);
-    if (act) out[p] = key;
-  };
-  stream_tile_sid<K>(
-      buf, sid, t0, t1,
-      [&](const K (&k)[N], const uint32_t (&s)[N]) {
+  stream_tile_vec<K>(
+      buf, t0, t1,
+      [&](const K (&k)[N]) {
         uint32_t b[N];
+        pm.template buckets<N>(k, b);
+        uint32_t sl[N];
 #pragma unroll
-        for (int j = 0; j < N; ++j) b[j] = pm.bucket(s[j], k[j]);
+        for (int j = 0; j < N; ++j) sl[j] = ct.slot[b[j]];
 #pragma unroll
-        for (int j = 0; j < N; ++j) put(k[j], b[j]);
+        for (int j = 0; j < N; ++j) {
+          bool act = sl[j] != kNone16;
+          uint32_t p = lds_claim(ct.cursor, sl[j], act);
+          if (act) out[p] = k[j];
+        }
       },
-      [&](K key, uint32_t s) { put(key, pm.bucket(s, key)); });
+      [&](K key) {
+        uint32_t s = ct.slot[pm.bucket(key)];
+        bool act = s != kNone16;
+        uint32_t p = lds_claim(ct.cursor, s, act);
+        if (act) out[p] = key;
+      });
 }
 
 template <class K>
