@@ -212,7 +212,7 @@ struct Prof {
 enum Buf {
   bBufA, bBufB, bRows, bTotals, bRanges0, bRanges1, bRangesFirst, bCtl, bKs, bKRank, bKState, bKAux, bKList,
   bSlotOfBucket, bSlotDst, bSlotBucket, bActBucket, bActCls, bActDst, bActSlot, bActKlo, bActKhi,
-  bSmallStart, bSmallLen, bSmallKStart, bSmallKFill, bWgFirst, bOut, bLut, bSmallKlo, bSmallKhi, bParams1, bCount
+  bSmallStart, bSmallLen, bSmallKStart, bSmallKFill, bWgFirst, bOut, bLut, bSmallKlo, bSmallKhi, bParams1, bActCnt, bCount
 };
 static_assert(bCount <= kMaxBuffers, "workspace layout table too small");
 
@@ -242,6 +242,7 @@ static Layout make_layout(uint64_t n, uint32_t nk, const Config& c) {
   sz[bSlotBucket] = Bm * 4;
   sz[bActBucket] = Bm * 4;
   sz[bActCls] = Bm * 4;
+  sz[bActCnt] = Bm * 4;
   sz[bActDst] = Bm * 4;
   sz[bActSlot] = Bm * 4;
   sz[bActKlo] = Bm * sizeof(K);
@@ -457,8 +458,8 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
         prof.end();
       }
       HIP_TRY(hipGetLastError());
-      prof.begin(GGMS_KERNEL_ROWSCAN, (int)pass);
-      k_rowscan<<<(B_bound + 63) / 64, 1024, 0, st>>>(rows, cp, totals);
+      prof.begin(GGMS_KERNEL_COLSUM, (int)pass);
+      k_colsum<<<(B_bound + 63) / 64, 1024, 0, st>>>(rows, cp, totals);
       prof.end();
       HIP_TRY(hipGetLastError());
       // plan
@@ -477,6 +478,7 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
       pl.slot_bucket = (uint32_t*)P(bSlotBucket);
       pl.act_bucket = (uint32_t*)P(bActBucket);
       pl.act_cls = (uint32_t*)P(bActCls);
+      pl.act_cnt = (uint32_t*)P(bActCnt);
       pl.act_dst = (uint32_t*)P(bActDst);
       pl.act_slot = (uint32_t*)P(bActSlot);
       pl.act_klo = (K*)P(bActKlo);
@@ -637,7 +639,7 @@ static ggms_status run_first_pass(ggms_ctx* ctx, const T* d_vec, uint64_t n64, c
   set_lds_attr((const void*)k_count0<T>, lds);
   k_count0<T><<<G0, c.NT, lds, st>>>(d_vec, rng, (K*)P(bLut), ctl, rows);
   HIP_TRY(hipGetLastError());
-  k_rowscan<<<(c.B0_bound + 63) / 64, 1024, 0, st>>>(rows, ctl, totals);
+  k_colsum<<<(c.B0_bound + 63) / 64, 1024, 0, st>>>(rows, ctl, totals);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipMemcpyAsync(&hctl[0], ctl, ctlb, hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));

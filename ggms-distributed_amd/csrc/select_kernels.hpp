@@ -1,21 +1,21 @@
 // Selection kernels for gfx950 (wave64, 160 KB LDS per CU).
 //
-// One bucketing pass = count -> rowscan -> plan -> compact -> finish:
+// One bucketing pass = count -> colsum -> plan -> compact -> finish:
 //   k_plan0    one workgroup: sample, sort the sample in LDS, place pivots,
 //              build the first pass's range table (linear-slope splitter).
 //   k_count0 / k_countS
 //              fused "assign bucket + count": 128-bit loads, per-workgroup LDS
 //              histogram, wave-peeled atomics; writes one histogram row per
 //              workgroup.  No per-element bucket ids are stored.
-//   k_rowscan  column-wise exclusive prefix over the rows (per-workgroup write
-//              offsets) plus the column totals (the pass histogram).
+//   k_colsum   column totals of the rows (the pass histogram).
 //   k_plan     one workgroup: inclusive scan of the histogram, findKBuckets for
 //              every live k, dedupe into active buckets, classify each active
 //              bucket (single key / small / big), lay out the next buffer,
 //              re-rank ks, and build the next pass's ranges.
 //   k_compact0 / k_compactS
 //              recompute each element's bucket and copy active-bucket elements
-//              to their segment, at the workgroup's precomputed offset.
+//              to their segment, at offsets the workgroup claims with one
+//              global atomic per segment (pass 0: one for the whole tile).
 //   k_finish   one workgroup per small segment: bitonic sort in LDS and pick.
 //
 // Reference counterparts (DistributedSMOS/): assignSmartBucket_iterative
@@ -57,7 +57,8 @@ struct PassCtl {
   uint32_t nan_lo_bucket;  // bucket holding keys below key(-inf), or kNone32
   uint32_t nan_hi_bucket;  // bucket holding keys above key(+inf), or kNone32
   uint32_t bad_k;          // number of ks outside [1, n]
-  uint32_t pad[15];
+  uint32_t append;         // first-pass compaction: next free position of the survivor buffer
+  uint32_t pad[14];
 };
 
 struct Plan0Args {
@@ -426,38 +427,63 @@ template <class K> __device__ __forceinline__ uint32_t param_bucket(const MapPar
   return q.base + h + id;
 }
 
-// The per-range map parameters are stored as separate arrays (lo, mult,
-// base, and sh for 64-bit keys): lanes of a wave that read different ranges
-// then hit different LDS banks (range r sits in bank r mod 32 of each array),
-// where one packed record per range made ranges r and r + 8 collide.
+// The per-range map parameters are stored as separate arrays, so lanes of a
+// wave that read different ranges hit different LDS banks (one packed record
+// per range made ranges r and r + 8 collide).
+//   32-bit keys: lo, mult, and c = base*2^32 + adj - lo*mult (mod 2^64); the
+//     bucket is the high word of key*mult + c, one v_mad_u64_u32.  For a
+//     non-identity map that is base + mulhi(key - lo, mult) (the 64-bit sum is
+//     exact: (key - lo)*mult < nb*2^32).  An identity map (mult == 0: every key
+//     its own bucket, key - lo < nb) is stored as mult = adj = 2^32 - 1:
+//     (t + 1)(2^32 - 1) = t*2^32 + (2^32 - 1 - t) has high word t.
+//   64-bit keys: lo, mult, base, sh, as param_bucket.
+// lo also serves the binary search of the rare many-start cells.
 template <class K> struct Pass0Map {
   using G = LutGeom<K>;
   K* lut;  // level 1 followed by the level-2 tables
   K* plo;
   uint32_t* pmult;
-  uint32_t* pbase;
-  uint32_t* psh;
+  uint32_t* pbase;  // 64-bit keys
+  uint32_t* psh;    // 64-bit keys
+  uint64_t* pc;     // 32-bit keys
 
-  // LDS bytes of the parameter arrays for R ranges (<= R packed records)
+  // LDS bytes of the parameter arrays for R ranges (16 bytes per range for
+  // 32-bit keys, 20 for 64-bit keys), rounded to 16
   static __host__ __device__ size_t param_bytes(uint32_t R) {
     return (((size_t)R * (sizeof(K) + 12)) + 15) / 16 * 16;
   }
   static __host__ __device__ size_t bytes(uint32_t R) { return G::bytes() + param_bytes(R); }
+  // p must be 8-byte aligned
   __device__ void set_params(unsigned char* p, uint32_t R) {
     plo = reinterpret_cast<K*>(p);
     pmult = reinterpret_cast<uint32_t*>(p + (size_t)R * sizeof(K));
-    pbase = pmult + R;
-    psh = pbase + R;
+    if constexpr (sizeof(K) == 4) {
+      pc = reinterpret_cast<uint64_t*>(p + (size_t)R * 8);
+      pbase = nullptr;
+      psh = nullptr;
+    } else {
+      pbase = pmult + R;
+      psh = pbase + R;
+      pc = nullptr;
+    }
   }
   __device__ void carve(unsigned char* p, uint32_t R) {
     lut = reinterpret_cast<K*>(p);
     set_params(p + G::bytes(), R);
   }
-  __device__ __forceinline__ void store_param(uint32_t r, const Range<K>& g) {
+  // g: a Range<K> or a MapParam<K> (fields lo, mult, base, sh)
+  template <class P> __device__ __forceinline__ void store_param(uint32_t r, const P& g) {
     plo[r] = g.lo;
-    pmult[r] = g.mult;
-    pbase[r] = g.base;
-    psh[r] = g.sh;
+    if constexpr (sizeof(K) == 4) {
+      const uint32_t m = g.mult ? g.mult : 0xFFFFFFFFu;
+      const uint64_t adj = g.mult ? 0ull : 0xFFFFFFFFull;
+      pmult[r] = m;
+      pc[r] = ((uint64_t)g.base << 32) + adj - (uint64_t)(uint32_t)g.lo * m;
+    } else {
+      pmult[r] = g.mult;
+      pbase[r] = g.base;
+      psh[r] = g.sh;
+    }
   }
   __device__ void load_params(const Range<K>* __restrict__ g, uint32_t R) {
     for (uint32_t i = threadIdx.x; i < R; i += blockDim.x) store_param(i, g[i]);
@@ -466,14 +492,18 @@ template <class K> struct Pass0Map {
     for (uint32_t i = threadIdx.x; i < G::L1CELLS + G::L2TABLES * 64; i += blockDim.x) lut[i] = g_lut[i];
     load_params(g, R);
   }
-  // bucket of key in range r (branch-free, as param_bucket)
+  // bucket of key in range r (branch-free, equal to param_bucket)
   __device__ __forceinline__ uint32_t range_bucket(uint32_t r, K key) const {
-    K d = key - plo[r];
-    uint32_t t = sizeof(K) == 4 ? (uint32_t)d : (uint32_t)(d >> psh[r]);
-    uint32_t m = pmult[r];
-    uint32_t h = __umulhi(t, m);
-    uint32_t id = m == 0 ? t : 0u;
-    return pbase[r] + h + id;
+    if constexpr (sizeof(K) == 4) {
+      return (uint32_t)(((uint64_t)(uint32_t)key * pmult[r] + pc[r]) >> 32);
+    } else {
+      K d = key - plo[r];
+      uint32_t t = (uint32_t)(d >> psh[r]);
+      uint32_t m = pmult[r];
+      uint32_t h = __umulhi(t, m);
+      uint32_t id = m == 0 ? t : 0u;
+      return pbase[r] + h + id;
+    }
   }
   // Range of a key in a level-1 leaf entry: r0 (the bits above S1; the flag
   // bit is 0 for a leaf) plus one when the key's in-cell offset passes the
@@ -722,33 +752,37 @@ __global__ void __launch_bounds__(1024) k_countS(const K* __restrict__ buf, cons
 }
 
 // ---------------------------------------------------------------------------
-// k_rowscan: rows[G][B] -> exclusive prefix down each column (in place) and
-// the column totals.  64 columns per workgroup, waves split the rows.
+// k_colsum: rows[G][B] -> column totals (the pass histogram).  64 columns per
+// workgroup; each wave sums a band of rows with eight loads in flight.  The
+// rows themselves are left as raw per-workgroup counts: the compactions claim
+// their write offsets from them with global atomics.
 // ---------------------------------------------------------------------------
-static __global__ void __launch_bounds__(1024) k_rowscan(uint32_t* __restrict__ rows, const PassCtl* __restrict__ ctl,
-                                                  uint32_t* __restrict__ totals) {
+static __global__ void __launch_bounds__(1024) k_colsum(const uint32_t* __restrict__ rows, const PassCtl* __restrict__ ctl,
+                                                 uint32_t* __restrict__ totals) {
   __shared__ uint32_t wsum[16][64];
   const uint32_t G = ctl->G, B = ctl->B;
   const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
   const uint32_t col = blockIdx.x * 64 + lane;
   if (blockIdx.x * 64 >= B) return;
-  uint32_t rpw = (G + nw - 1) / nw;
-  uint32_t r0 = wv * rpw, r1 = r0 + rpw < G ? r0 + rpw : G;
   uint32_t s = 0;
-  if (col < B)
-    for (uint32_t r = r0; r < r1; ++r) s += rows[(size_t)r * B + col];
+  if (col < B) {
+    uint32_t r = wv;
+    for (; r + 7 * nw < G; r += 8 * nw) {
+      uint32_t v[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) v[j] = rows[(size_t)(r + j * nw) * B + col];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) s += v[j];
+    }
+    for (; r < G; r += nw) s += rows[(size_t)r * B + col];
+  }
   wsum[wv][lane] = s;
   __syncthreads();
-  uint32_t off = 0;
-  for (uint32_t j = 0; j < wv; ++j) off += wsum[j][lane];
-  if (wv == nw - 1 && col < B) totals[col] = off + s;
-  if (col < B)
-    for (uint32_t r = r0; r < r1; ++r) {
-      size_t idx = (size_t)r * B + col;
-      uint32_t v = rows[idx];
-      rows[idx] = off;
-      off += v;
-    }
+  if (wv == 0 && col < B) {
+    uint32_t t = 0;
+    for (uint32_t j = 0; j < nw; ++j) t += wsum[j][lane];
+    totals[col] = t;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -769,6 +803,7 @@ template <class K> struct PlanArgs {
   uint32_t* slot_bucket;     // [slots]
   uint32_t* act_bucket;      // [U]
   uint32_t* act_cls;
+  uint32_t* act_cnt;         // elements of each active bucket
   uint32_t* act_dst;
   uint32_t* act_slot;
   K* act_klo;
@@ -840,17 +875,19 @@ __global__ void __launch_bounds__(1024) k_plan(PlanArgs<typename KeyTraits<T>::K
     uint32_t tot;
     uint32_t off = block_excl_scan_u32(s, scratch, &tot);
     for (uint32_t i = b0; i < b1; ++i)
-      if (aidx[i]) aidx[i] = ++off;
+      if (aidx[i]) {
+        a.act_bucket[off] = i;
+        aidx[i] = ++off;
+      }
     if (t == 0) sh_tot[0] = tot;
+    __threadfence_block();
     __syncthreads();
   }
   const uint32_t U = sh_tot[0];
 
-  // P3: classify active buckets
-  for (uint32_t b = t; b < B; b += nt) {
-    uint32_t ai = aidx[b];
-    if (!ai) continue;
-    uint32_t q = ai - 1;
+  // P3: classify active buckets (one per thread, from the list P2 wrote)
+  for (uint32_t q = t; q < U; q += nt) {
+    uint32_t b = a.act_bucket[q];
     uint32_t c = a.totals[b];
     // owning range: last r with base <= b
     uint32_t lo = 0, hi = ctl.R - 1;
@@ -866,8 +903,8 @@ __global__ void __launch_bounds__(1024) k_plan(PlanArgs<typename KeyTraits<T>::K
     else if (a.merge_small) cls = kClsBig;
     else if (c <= a.fin_cap) cls = kClsSmall;
     else cls = a.force_finish ? kClsSmall : kClsBig;
-    a.act_bucket[q] = b;
     a.act_cls[q] = cls;
+    a.act_cnt[q] = c;
     a.act_klo[q] = klo;
     a.act_khi[q] = khi;
   }
@@ -880,7 +917,7 @@ __global__ void __launch_bounds__(1024) k_plan(PlanArgs<typename KeyTraits<T>::K
     uint32_t q0 = t * per, q1 = q0 + per < U ? q0 + per : U;
     uint32_t nbig = 0, cbig = 0, nsml = 0, csml = 0;
     for (uint32_t q = q0; q < q1; ++q) {
-      uint32_t cls = a.act_cls[q], c = a.totals[a.act_bucket[q]];
+      uint32_t cls = a.act_cls[q], c = a.act_cnt[q];
       if (cls == kClsBig) { nbig++; cbig += c; }
       else if (cls == kClsSmall) { nsml++; csml += c; }
     }
@@ -890,7 +927,7 @@ __global__ void __launch_bounds__(1024) k_plan(PlanArgs<typename KeyTraits<T>::K
     uint32_t e2 = block_excl_scan_u32(nsml, scratch, &T2);
     uint32_t e3 = block_excl_scan_u32(csml, scratch, &T3);
     for (uint32_t q = q0; q < q1; ++q) {
-      uint32_t cls = a.act_cls[q], c = a.totals[a.act_bucket[q]];
+      uint32_t cls = a.act_cls[q], c = a.act_cnt[q];
       if (cls == kClsBig) {
         a.act_slot[q] = e0++;
         a.act_dst[q] = e1;
@@ -920,7 +957,7 @@ __global__ void __launch_bounds__(1024) k_plan(PlanArgs<typename KeyTraits<T>::K
     uint32_t s = a.act_slot[q];
     if (s == kNone32) continue;
     uint32_t b = a.act_bucket[q];
-    uint32_t c = a.totals[b];
+    uint32_t c = a.act_cnt[q];
     uint32_t dst = a.act_dst[q];
     a.slot_of_bucket[b] = (uint16_t)s;
     a.slot_dst[s] = dst;
@@ -1069,9 +1106,11 @@ __global__ void __launch_bounds__(1024) k_plan(PlanArgs<typename KeyTraits<T>::K
 
 // ---------------------------------------------------------------------------
 // Compaction: copy every element of an active (small or big) bucket to its
-// segment.  Workgroup w starts each segment cursor at
-//   slot_dst[s] + (elements of that bucket in workgroups < w)
-// so the segments come out contiguous without any global atomics.
+// segment.  Segment s occupies [seg_cursor[s] at plan time, + count); workgroup
+// w claims the run for its own elements of that bucket with one atomicAdd of
+// its histogram-row count (segments come out contiguous; the order of the
+// workgroups' runs inside a segment is whatever the atomics give, which no
+// later step depends on).
 // ---------------------------------------------------------------------------
 struct CompactTables {
   uint16_t* slot;    // [B]
@@ -1080,14 +1119,16 @@ struct CompactTables {
 
 __device__ __forceinline__ void compact_init(CompactTables& ct, unsigned char* p, uint32_t B, uint32_t U_slots,
                                              const uint16_t* __restrict__ slot_of_bucket,
-                                             const uint32_t* __restrict__ slot_dst,
+                                             uint32_t* __restrict__ seg_cursor,
                                              const uint32_t* __restrict__ slot_bucket,
-                                             const uint32_t* __restrict__ row_prefix) {
+                                             const uint32_t* __restrict__ row) {
   ct.slot = reinterpret_cast<uint16_t*>(p);
   ct.cursor = reinterpret_cast<uint32_t*>(p + ((size_t)B * 2 + 15) / 16 * 16);
   for (uint32_t i = threadIdx.x; i < B; i += blockDim.x) ct.slot[i] = slot_of_bucket[i];
-  for (uint32_t s = threadIdx.x; s < U_slots; s += blockDim.x)
-    ct.cursor[s] = slot_dst[s] + row_prefix[slot_bucket[s]];
+  for (uint32_t s = threadIdx.x; s < U_slots; s += blockDim.x) {
+    uint32_t c = row[slot_bucket[s]];
+    ct.cursor[s] = c ? atomicAdd(&seg_cursor[s], c) : 0u;
+  }
 }
 
 __host__ __device__ __forceinline__ size_t compact_lds_bytes(uint32_t B, uint32_t U_slots) {
@@ -1095,16 +1136,16 @@ __host__ __device__ __forceinline__ size_t compact_lds_bytes(uint32_t B, uint32_
 }
 
 // First-pass compaction, append layout: workgroup w writes the survivors of
-// its tile (elements of any active bucket) contiguously to
-//   [R_w, R_w + n_w),  R_w = sum over active buckets b of rows_prefix[w][b]
-// (survivors of all earlier workgroups; the sum telescopes, so no atomics).
-// Each wave claims a run for its survivors with one LDS atomic and writes it
-// with consecutive lanes, so HBM sees full lines instead of one partial line
-// per survivor.  The next pass finds each element's range from its key.
+// its tile (elements of any active bucket) contiguously to [R_w, R_w + n_w),
+// n_w = sum over active buckets of its histogram row, R_w claimed with one
+// global atomicAdd on the pass's append cursor.  Each wave claims a run for its
+// survivors with one LDS atomic and writes it with consecutive lanes, so HBM
+// sees full lines instead of one partial line per survivor.  The next pass
+// finds each element's range from its key, so the order of the runs is free.
 template <class T>
 __global__ void __launch_bounds__(1024) k_compact0(const T* __restrict__ x, const Range<typename KeyTraits<T>::K>* __restrict__ ranges,
                                                    const typename KeyTraits<T>::K* __restrict__ lut,
-                                                   const PassCtl* __restrict__ ctl, const uint32_t* __restrict__ rows,
+                                                   PassCtl* __restrict__ ctl, const uint32_t* __restrict__ rows,
                                                    const uint16_t* __restrict__ slot_of_bucket,
                                                    const uint32_t* __restrict__ slot_bucket,
                                                    typename KeyTraits<T>::K* __restrict__ out) {
@@ -1127,7 +1168,7 @@ __global__ void __launch_bounds__(1024) k_compact0(const T* __restrict__ x, cons
   {
     uint32_t tot;
     (void)block_excl_scan_u32(acc, part, &tot);
-    if (threadIdx.x == 0) cursor = tot;
+    if (threadIdx.x == 0) cursor = tot ? atomicAdd(&ctl->append, tot) : 0u;
   }
   __syncthreads();
   uint32_t t0 = blockIdx.x * tile;
@@ -1227,13 +1268,7 @@ __device__ __forceinline__ void pass1_map_init(Pass1Map<K>& pm, unsigned char* p
   pm.p1_lds = p1_in_lds;
   if (p1_in_lds) {
     pm.p1s.set_params(q, R1);
-    for (uint32_t i = threadIdx.x; i < R1; i += blockDim.x) {
-      const MapParam<K> g = p1_g[i];
-      pm.p1s.plo[i] = g.lo;
-      pm.p1s.pmult[i] = g.mult;
-      pm.p1s.pbase[i] = g.base;
-      pm.p1s.psh[i] = g.sh;
-    }
+    for (uint32_t i = threadIdx.x; i < R1; i += blockDim.x) pm.p1s.store_param(i, p1_g[i]);
   }
 }
 
@@ -1275,7 +1310,7 @@ __global__ void __launch_bounds__(1024) k_compact1(const K* __restrict__ buf, co
                                                    uint32_t p1_in_lds, const PassCtl* __restrict__ ctl,
                                                    const uint32_t* __restrict__ rows,
                                                    const uint16_t* __restrict__ slot_of_bucket,
-                                                   const uint32_t* __restrict__ slot_dst,
+                                                   uint32_t* __restrict__ slot_dst,
                                                    const uint32_t* __restrict__ slot_bucket, K* __restrict__ out) {
   extern __shared__ __align__(16) unsigned char smem[];
   const uint32_t B = ctl->B, n = ctl->n, tile = ctl->tile, U_slots = ctl->U_slots;
@@ -1317,7 +1352,7 @@ __global__ void __launch_bounds__(1024) k_compactS(const K* __restrict__ buf, co
                                                    const uint32_t* __restrict__ wg_first, const PassCtl* __restrict__ ctl,
                                                    const uint32_t* __restrict__ rows,
                                                    const uint16_t* __restrict__ slot_of_bucket,
-                                                   const uint32_t* __restrict__ slot_dst,
+                                                   uint32_t* __restrict__ slot_dst,
                                                    const uint32_t* __restrict__ slot_bucket, K* __restrict__ out) {
   extern __shared__ __align__(16) unsigned char smem[];
   const uint32_t B = ctl->B, n = ctl->n, tile = ctl->tile, U_slots = ctl->U_slots;
@@ -1390,8 +1425,47 @@ __device__ K radix_select_lds(const K* __restrict__ s, uint32_t len, K klo, K kh
   return prefix;
 }
 
+// Loads buf[st, st + len) into s[0, len) with 16-byte loads, up to kSegLoadVecs
+// per lane in flight (a segment of the finish capacity is one round trip).
+// The vectors are aligned down/up to 16 bytes; the keys outside the segment
+// they carry are dropped.  Every buffer this reads is allocated in 256-byte
+// multiples from a 256-byte aligned base, so the rounded vectors stay inside it.
+constexpr int kSegLoadVecs = 4;
+
+template <class K>
+__device__ __forceinline__ void load_segment_lds(K* __restrict__ s, const K* __restrict__ buf, uint32_t st, uint32_t len) {
+  constexpr uint32_t N = 16 / sizeof(K);
+  const uint32_t head = st & (N - 1);
+  const uint4* src = reinterpret_cast<const uint4*>(buf + (st - head));
+  const uint32_t nv = (head + len + N - 1) / N;
+  auto put = [&](uint32_t i, const uint4& v) {
+    K e[N];
+    __builtin_memcpy(e, &v, 16);
+#pragma unroll
+    for (uint32_t q = 0; q < N; ++q) {
+      uint32_t idx = i * N + q - head;  // wraps above len for the dropped head keys
+      if (idx < len) s[idx] = e[q];
+    }
+  };
+  uint4 v[kSegLoadVecs];
+#pragma unroll
+  for (int j = 0; j < kSegLoadVecs; ++j) {
+    uint32_t i = threadIdx.x + (uint32_t)j * blockDim.x;
+    if (i < nv) v[j] = src[i];
+  }
+#pragma unroll
+  for (int j = 0; j < kSegLoadVecs; ++j) {
+    uint32_t i = threadIdx.x + (uint32_t)j * blockDim.x;
+    if (i < nv) put(i, v[j]);
+  }
+  for (uint32_t i = threadIdx.x + kSegLoadVecs * blockDim.x; i < nv; i += blockDim.x) put(i, src[i]);
+}
+
+// Eight waves per SIMD (two workgroups per CU, the LDS allows it) caps the
+// scalar registers: left alone the compiler takes 91 SGPRs, which admits only
+// one 1024-thread workgroup per CU.
 template <class T>
-__global__ void __launch_bounds__(1024) k_finish(const typename KeyTraits<T>::K* __restrict__ buf, const PassCtl* __restrict__ ctl,
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8))) k_finish(const typename KeyTraits<T>::K* __restrict__ buf, const PassCtl* __restrict__ ctl,
                                                  const uint32_t* __restrict__ small_start,
                                                  const uint32_t* __restrict__ small_len,
                                                  const typename KeyTraits<T>::K* __restrict__ small_klo,
@@ -1416,12 +1490,16 @@ __global__ void __launch_bounds__(1024) k_finish(const typename KeyTraits<T>::K*
     if (len <= len_lo || len > len_hi) continue;  // another launch's size class
     uint32_t q0 = small_kstart[j], q1 = small_kstart[j + 1];
     if (q1 - q0 <= kSelectMaxKs) {
-      for (uint32_t i = threadIdx.x; i < len; i += blockDim.x) s[i] = buf[st + i];
-      __syncthreads();
       K klo = small_klo[j], khi = small_khi[j];
+      // the first k and its rank are fetched while the segment loads
+      uint32_t k0 = q1 > q0 ? klist[q0] : 0u;
+      uint32_t r0 = q1 > q0 ? k_rank[k0] : 0u;
+      load_segment_lds<K>(s, buf, st, len);
+      __syncthreads();
       for (uint32_t q = q0; q < q1; ++q) {
-        uint32_t k = klist[q];
-        K key = radix_select_lds<K>(s, len, klo, khi, k_rank[k], hist, bcast);
+        uint32_t k = q == q0 ? k0 : klist[q];
+        uint32_t r = q == q0 ? r0 : k_rank[k];
+        K key = radix_select_lds<K>(s, len, klo, khi, r, hist, bcast);
         if (threadIdx.x == 0) {
           out[k] = KeyTraits<T>::dec(key);
           k_state[k] = kDone;
@@ -1429,7 +1507,8 @@ __global__ void __launch_bounds__(1024) k_finish(const typename KeyTraits<T>::K*
       }
     } else {
       uint32_t M = pow2ceil_u32(len);
-      for (uint32_t i = threadIdx.x; i < M; i += blockDim.x) s[i] = i < len ? buf[st + i] : KeyMax<K>::v;
+      load_segment_lds<K>(s, buf, st, len);
+      for (uint32_t i = len + threadIdx.x; i < M; i += blockDim.x) s[i] = KeyMax<K>::v;
       __syncthreads();
       bitonic_sort_lds<K>(s, M);
       for (uint32_t q = q0 + threadIdx.x; q < q1; q += blockDim.x) {

@@ -13,7 +13,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libggms.so")
 
 GGMS_MAX_PASS_STATS = 64
-KERNEL_KINDS = ["plan0", "count0", "countS", "rowscan", "plan", "compact0", "compactS", "finish", "other"]
+KERNEL_KINDS = ["plan0", "count0", "countS", "colsum", "plan", "compact0", "compactS", "finish", "other"]
 
 
 class Opts(C.Structure):

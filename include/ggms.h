@@ -70,7 +70,7 @@ typedef struct {
 #define GGMS_MAX_PASS_STATS 64
 /* kernel kinds timed when ggms_opts.profile is set */
 enum {
-  GGMS_KERNEL_PLAN0 = 0, GGMS_KERNEL_COUNT0 = 1, GGMS_KERNEL_COUNTS = 2, GGMS_KERNEL_ROWSCAN = 3, GGMS_KERNEL_PLAN = 4,
+  GGMS_KERNEL_PLAN0 = 0, GGMS_KERNEL_COUNT0 = 1, GGMS_KERNEL_COUNTS = 2, GGMS_KERNEL_COLSUM = 3, GGMS_KERNEL_PLAN = 4,
   GGMS_KERNEL_COMPACT0 = 5, GGMS_KERNEL_COMPACTS = 6, GGMS_KERNEL_FINISH = 7, GGMS_KERNEL_OTHER = 8,
   GGMS_KERNEL_KINDS = 9
 };
