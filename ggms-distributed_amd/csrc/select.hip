@@ -131,7 +131,9 @@ struct ggms_ctx {
   size_t max_lds = 65536;
   Workspace ws;
   void* pinned = nullptr;  // host staging (ctl blocks, ks, outputs)
+  void* pinned_dev = nullptr;  // the same memory as the device addresses it
   size_t pinned_bytes = 0;
+  hipEvent_t plan_ev = nullptr;  // recorded after each pass's plan kernel
   ggms_stats stats{};
   std::vector<hipEvent_t> events;  // timing events (opts.profile)
   std::mutex mu;
@@ -145,8 +147,16 @@ static ggms_status ensure_pinned(ggms_ctx* c, size_t bytes) {
   c->pinned = nullptr;
   c->pinned_bytes = 0;
   size_t b = std::max<size_t>(bytes, 1 << 16);
-  if (hipHostMalloc(&c->pinned, b, hipHostMallocDefault) != hipSuccess) return GGMS_ENOMEM;
+  // coherent: the plan kernel writes control blocks here for the host to read
+  // as soon as the kernel has finished
+  if (hipHostMalloc(&c->pinned, b, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess) return GGMS_ENOMEM;
+  if (hipHostGetDevicePointer(&c->pinned_dev, c->pinned, 0) != hipSuccess) {
+    (void)hipHostFree(c->pinned);
+    c->pinned = nullptr;
+    return GGMS_EDEVICE;
+  }
   c->pinned_bytes = b;
+  if (!c->plan_ev && hipEventCreateWithFlags(&c->plan_ev, hipEventDisableTiming) != hipSuccess) return GGMS_EDEVICE;
   return GGMS_OK;
 }
 
@@ -212,7 +222,7 @@ struct Prof {
 enum Buf {
   bBufA, bBufB, bRows, bTotals, bRanges0, bRanges1, bRangesFirst, bCtl, bKs, bKRank, bKState, bKAux, bKList,
   bSlotOfBucket, bSlotDst, bSlotBucket, bActBucket, bActCls, bActDst, bActSlot, bActKlo, bActKhi,
-  bSmallStart, bSmallLen, bSmallKStart, bSmallKFill, bWgFirst, bOut, bLut, bSmallKlo, bSmallKhi, bParams1, bActCnt, bCount
+  bSmallStart, bSmallLen, bSmallKStart, bSmallKFill, bWgFirst, bOut, bLut, bSmallKlo, bSmallKhi, bParams1, bActCnt, bRank0, bCount
 };
 static_assert(bCount <= kMaxBuffers, "workspace layout table too small");
 
@@ -243,6 +253,7 @@ static Layout make_layout(uint64_t n, uint32_t nk, const Config& c) {
   sz[bActBucket] = Bm * 4;
   sz[bActCls] = Bm * 4;
   sz[bActCnt] = Bm * 4;
+  sz[bRank0] = (size_t)rank_word_count((uint32_t)Bm) * 8;
   sz[bActDst] = Bm * 4;
   sz[bActSlot] = Bm * 4;
   sz[bActKlo] = Bm * sizeof(K);
@@ -283,6 +294,8 @@ static size_t count_lds0_total(const Config& c) {
 // LDS ceiling of one workgroup on gfx950 (160 KB per CU; static LDS of the
 // kernels stays under 2 KB).
 constexpr size_t kLdsLimit = 158 * 1024;
+// dynamic LDS that leaves room for only one 1024-thread workgroup per CU
+constexpr size_t kLdsOneWgPerCu = 81 * 1024;
 
 static void set_lds_attr(const void* fn, size_t bytes) {
   if (bytes > 65536) (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
@@ -312,7 +325,7 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
   ggms_status s = ensure_ws(ctx, L.total);
   if (s) return s;
   const size_t ctlb = sizeof(PassCtl);
-  s = ensure_pinned(ctx, 4 * ctlb + (size_t)nk * 8 + 4096);
+  s = ensure_pinned(ctx, 6 * ctlb + 256 + (size_t)nk * 8 + 4096);
   if (s) return s;
   char* W = (char*)ctx->ws.base;
   auto P = [&](int b) { return (void*)(W + L.off[b]); };
@@ -330,9 +343,18 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
   uint32_t* klist = (uint32_t*)P(bKList);
   T* d_out = out_on_device ? (T*)out : (T*)P(bOut);
 
+  // pinned host memory: ctl blocks [0, 4) for copies, the plan's mirror of the
+  // current and next control blocks [4, 6), the finish's error flag, then ks in
+  // and outputs out
   char* H = (char*)ctx->pinned;
-  PassCtl* hctl = (PassCtl*)H;                 // two blocks: this pass, next pass
-  char* hdata = H + 4 * ctlb;                  // ks in, outputs out
+  char* Hd = (char*)ctx->pinned_dev;
+  PassCtl* hctl = (PassCtl*)H;
+  PassCtl* hmirror = hctl + 4;
+  PassCtl* dmirror = (PassCtl*)(Hd + 4 * ctlb);
+  volatile uint32_t* hflag = (volatile uint32_t*)(H + 6 * ctlb);
+  uint32_t* dflag = (uint32_t*)(Hd + 6 * ctlb);
+  char* hdata = H + 6 * ctlb + 256;
+  *hflag = 0;
 
   // control block 0
   PassCtl c0{};
@@ -380,13 +402,13 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
     prof.begin(GGMS_KERNEL_FINISH, 0);
     k_finish<T><<<1, 1024, fin_lds, st>>>(bufA, ctl, (uint32_t*)P(bSmallStart), (uint32_t*)P(bSmallLen),
                                           (K*)P(bSmallKlo), (K*)P(bSmallKhi), (uint32_t*)P(bSmallKStart), klist, k_rank, k_state, d_out,
-                                          0u, c.fin_cap, c.fin_cap, &ctl->pad[0]);
+                                          0u, c.fin_cap, c.fin_cap, dflag);
     prof.end();
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(&hctl[0], ctl, ctlb, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     if (hctl[0].bad_k) return GGMS_EINVAL;
-    if (hctl[0].pad[0]) return GGMS_EDEVICE;
+    if (*hflag) return GGMS_EDEVICE;
     stt.passes = 0;
     stt.finished_segments = 1;
   } else {
@@ -409,7 +431,7 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
     HIP_TRY(hipGetLastError());
 
     uint32_t G = G0, B_bound = c.B0_bound, R_bound = c.R0_bound;
-    uint32_t B0 = c.B0_bound, R0 = c.R0_bound;  // exact after pass 0
+    uint32_t B0 = c.B0_bound, R0 = c.R0_bound, L2 = LutGeom<K>::L2TABLES;  // exact after pass 0
     uint16_t* slot_tab[2] = {(uint16_t*)P(bSlotOfBucket),
                              (uint16_t*)P(bSlotOfBucket) + (std::max<size_t>(c.B0_bound, c.B_cap) + 64)};
     uint32_t U_slot_bound = std::min<uint32_t>(nk, B_bound);
@@ -429,8 +451,8 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
       // pass-1 kernel past 112 KB (then they are read from global memory)
       uint32_t p1_lds = 0;
       if (pass == 1) {
-        size_t cnt = ((size_t)B_bound * 4 + 15) / 16 * 16 + pass1_map_bytes<K>(B0, R0, R_bound);
-        size_t cmp = compact_lds_bytes(B_bound, U_slot_bound + 4) + pass1_map_bytes<K>(B0, R0, R_bound);
+        size_t cnt = ((size_t)B_bound * 4 + 15) / 16 * 16 + pass1_map_bytes<K>(B0, R0, L2, R_bound);
+        size_t cmp = compact_lds_bytes(B_bound, U_slot_bound + 4) + pass1_map_bytes<K>(B0, R0, L2, R_bound);
         p1_lds = std::max(cnt, cmp) <= 112 * 1024 ? 1u : 0u;
       }
       // count
@@ -442,11 +464,11 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
         k_count0<T><<<G, NT, lds, st>>>(d_vec, rc, (K*)P(bLut), cp, rows);
         prof.end();
       } else if (pass == 1) {
-        size_t lds = ((size_t)B_bound * 4 + 15) / 16 * 16 + pass1_map_bytes<K>(B0, R0, p1_lds ? R_bound : 0);
+        size_t lds = ((size_t)B_bound * 4 + 15) / 16 * 16 + pass1_map_bytes<K>(B0, R0, L2, p1_lds ? R_bound : 0);
         LDS_CHECK(lds);
         set_lds_attr((const void*)k_count1<K>, lds);
         prof.begin(GGMS_KERNEL_COUNTS, (int)pass);
-        k_count1<K><<<G, NT, lds, st>>>(src, ctl, (K*)P(bLut), rng_first, slot_tab[0],
+        k_count1<K><<<G, NT, lds, st>>>(src, ctl, (K*)P(bLut), rng_first, (uint64_t*)P(bRank0),
                                         (MapParam<K>*)P(bParams1), p1_lds, cp, rows);
         prof.end();
       } else {
@@ -500,6 +522,8 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
       pl.force_finish = (pass + 1 >= c.max_passes) ? 1u : c.force_finish;
       pl.merge_small = pass == 0 ? 1u : 0u;
       pl.next_params = pass == 0 ? (MapParam<K>*)P(bParams1) : nullptr;
+      pl.rank_words = pass == 0 ? (uint64_t*)P(bRank0) : nullptr;
+      pl.mirror = dmirror;
       size_t lds_plan = (size_t)B_bound * 8;
       LDS_CHECK(lds_plan);
       set_lds_attr((const void*)k_plan<T>, lds_plan);
@@ -507,6 +531,7 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
       k_plan<T><<<1, 1024, lds_plan, st>>>(pl);
       prof.end();
       HIP_TRY(hipGetLastError());
+      HIP_TRY(hipEventRecord(ctx->plan_ev, st));
       // compaction
       if (pass == 0) {
         // dynamic part: map parameters then the slot table (cell table is static LDS)
@@ -518,11 +543,15 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
                                           dst);
         prof.end();
       } else if (pass == 1) {
-        size_t lds = compact_lds_bytes(B_bound, U_slot_bound + 4) + pass1_map_bytes<K>(B0, R0, p1_lds ? R_bound : 0);
+        size_t lds = compact_lds_bytes(B_bound, U_slot_bound + 4) + pass1_map_bytes<K>(B0, R0, L2, p1_lds ? R_bound : 0);
+        // one workgroup per CU: its writes scatter over every k-bucket segment,
+        // and twice the workgroups in flight keep twice the partial HBM lines
+        // open (measured 0.197 ms with two per CU, 0.154 ms with one)
+        lds = std::max<size_t>(lds, kLdsOneWgPerCu);
         LDS_CHECK(lds);
         set_lds_attr((const void*)k_compact1<K>, lds);
         prof.begin(GGMS_KERNEL_COMPACTS, (int)pass);
-        k_compact1<K><<<G, NT, lds, st>>>(src, ctl, (K*)P(bLut), rng_first, slot_tab[0],
+        k_compact1<K><<<G, NT, lds, st>>>(src, ctl, (K*)P(bLut), rng_first, (uint64_t*)P(bRank0),
                                           (MapParam<K>*)P(bParams1), p1_lds, cp, rows,
                                           pl.slot_of_bucket, pl.slot_dst, pl.slot_bucket, dst);
         prof.end();
@@ -540,19 +569,23 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
       prof.begin(GGMS_KERNEL_FINISH, (int)pass);
       k_finish<T><<<fin_grid, 1024, fin_lds, st>>>(dst, cp, pl.small_start, pl.small_len, pl.small_klo,
                                                    pl.small_khi, pl.small_kstart, klist, k_rank, k_state, d_out,
-                                                   0u, c.fin_cap, c.fin_cap, &cp->pad[0]);
+                                                   0u, c.fin_cap, c.fin_cap, dflag);
       prof.end();
       HIP_TRY(hipGetLastError());
       }
-      HIP_TRY(hipMemcpyAsync(&hctl[0], cp, 2 * ctlb, hipMemcpyDeviceToHost, st));
-      HIP_TRY(hipStreamSynchronize(st));
-      const PassCtl& h = hctl[0];
-      const PassCtl& hn = hctl[1];
+      // the next pass is planned from the control blocks the plan kernel wrote
+      // to host memory, while this pass's compaction and finish still run
+      HIP_TRY(hipEventSynchronize(ctx->plan_ev));
+      const PassCtl h = hmirror[0];
+      const PassCtl hn = hmirror[1];
+      if (pass == 0 && (h.bad_k || h.nan_count)) {
+        HIP_TRY(hipStreamSynchronize(st));
+        return h.bad_k ? GGMS_EINVAL : GGMS_ENAN;
+      }
       if (pass == 0) {
-        if (h.bad_k) return GGMS_EINVAL;
-        if (h.nan_count) return GGMS_ENAN;
         B0 = h.B;
         R0 = h.R;
+        L2 = h.l2_used;
       }
       if (pass < GGMS_MAX_PASS_STATS) {
         stt.n_in[pass] = h.n;
@@ -562,9 +595,11 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
       }
       stt.passes = pass + 1;
       stt.finished_segments += h.U_small;
-      if (h.pad[0]) return GGMS_EDEVICE;
       if (h.N_big == 0 || h.U_big == 0) break;
-      if (pass + 1 >= c.max_passes) return GGMS_EDEVICE;  // cannot happen: the last pass forces finishing
+      if (pass + 1 >= c.max_passes) {  // cannot happen: the last pass forces finishing
+        HIP_TRY(hipStreamSynchronize(st));
+        return GGMS_EDEVICE;
+      }
       G = hn.G;
       B_bound = std::max<uint32_t>(hn.B, 1);
       R_bound = hn.R;
@@ -578,6 +613,7 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
   } else {
     HIP_TRY(hipStreamSynchronize(st));
   }
+  if (*hflag) return GGMS_EDEVICE;  // a segment over the finish capacity (never expected)
   prof.collect(stt);
   return GGMS_OK;
 }
@@ -763,6 +799,7 @@ void ggms_ctx_destroy(ggms_ctx* c) {
   (void)hipSetDevice(c->device);
   if (c->ws.base) (void)hipFree(c->ws.base);
   if (c->pinned) (void)hipHostFree(c->pinned);
+  if (c->plan_ev) (void)hipEventDestroy(c->plan_ev);
   for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
   (void)hipSetDevice(cur);
   delete c;

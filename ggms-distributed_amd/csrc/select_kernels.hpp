@@ -58,7 +58,8 @@ struct PassCtl {
   uint32_t nan_hi_bucket;  // bucket holding keys above key(+inf), or kNone32
   uint32_t bad_k;          // number of ks outside [1, n]
   uint32_t append;         // first-pass compaction: next free position of the survivor buffer
-  uint32_t pad[14];
+  uint32_t l2_used;        // first pass: level-2 tables of the cell table in use
+  uint32_t pad[13];
 };
 
 struct Plan0Args {
@@ -198,6 +199,9 @@ __device__ void bitonic_sort_lds(K* s, uint32_t M) {
   }
 }
 
+// words of a slot rank table (32 buckets per 64-bit word)
+__host__ __device__ __forceinline__ uint32_t rank_word_count(uint32_t B) { return (B + 31) / 32; }
+
 __device__ __forceinline__ uint32_t pow2ceil_u32(uint32_t v) {
   if (v <= 1) return 1;
   return 1u << (32 - __builtin_clz(v - 1));
@@ -239,6 +243,15 @@ template <> struct Vec16<uint64_t> {
   }
 };
 
+// 16-byte streaming load with the nontemporal hint: each element is read once,
+// so the caches need not keep it (measured on one MI355X: a plain 1 GiB read
+// sweep 6.6 TB/s, with the hint 7.0 TB/s).
+__device__ __forceinline__ uint4 ld_stream(const uint4* p) {
+  typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+  const u32x4 t = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(p));
+  return make_uint4(t.x, t.y, t.z, t.w);
+}
+
 // Streams elements [t0, t1) of x (16-byte aligned base, t0 a multiple of the
 // vector width) through f(value, position).  Four 16-byte loads in flight per
 // lane.
@@ -250,10 +263,10 @@ __device__ __forceinline__ void stream_tile(const V* __restrict__ x, uint32_t t0
   uint32_t v1 = t1 / N;
   uint32_t v = t0 / N + threadIdx.x;
   for (; v + 3 * nt < v1; v += 4 * nt) {
-    uint4 q0 = xv[v];
-    uint4 q1 = xv[v + nt];
-    uint4 q2 = xv[v + 2 * nt];
-    uint4 q3 = xv[v + 3 * nt];
+    uint4 q0 = ld_stream(xv + v);
+    uint4 q1 = ld_stream(xv + v + nt);
+    uint4 q2 = ld_stream(xv + v + 2 * nt);
+    uint4 q3 = ld_stream(xv + v + 3 * nt);
     V e[4][N];
     Vec16<V>::unpack(q0, e[0]);
     Vec16<V>::unpack(q1, e[1]);
@@ -265,7 +278,7 @@ __device__ __forceinline__ void stream_tile(const V* __restrict__ x, uint32_t t0
       for (int j = 0; j < N; ++j) f(e[u][j], (v + u * nt) * N + j);
   }
   for (; v < v1; v += nt) {
-    uint4 q0 = xv[v];
+    uint4 q0 = ld_stream(xv + v);
     V e[N];
     Vec16<V>::unpack(q0, e);
 #pragma unroll
@@ -286,10 +299,10 @@ __device__ __forceinline__ void stream_tile_vec(const V* __restrict__ x, uint32_
   uint32_t v1 = t1 / N;
   uint32_t v = t0 / N + threadIdx.x;
   for (; v + 3 * nt < v1; v += 4 * nt) {
-    uint4 q0 = xv[v];
-    uint4 q1 = xv[v + nt];
-    uint4 q2 = xv[v + 2 * nt];
-    uint4 q3 = xv[v + 3 * nt];
+    uint4 q0 = ld_stream(xv + v);
+    uint4 q1 = ld_stream(xv + v + nt);
+    uint4 q2 = ld_stream(xv + v + 2 * nt);
+    uint4 q3 = ld_stream(xv + v + 3 * nt);
     V e[N];
     Vec16<V>::unpack(q0, e);
     fv(e);
@@ -301,7 +314,7 @@ __device__ __forceinline__ void stream_tile_vec(const V* __restrict__ x, uint32_
     fv(e);
   }
   for (; v < v1; v += nt) {
-    uint4 q0 = xv[v];
+    uint4 q0 = ld_stream(xv + v);
     V e[N];
     Vec16<V>::unpack(q0, e);
     fv(e);
@@ -453,6 +466,9 @@ template <class K> struct Pass0Map {
     return (((size_t)R * (sizeof(K) + 12)) + 15) / 16 * 16;
   }
   static __host__ __device__ size_t bytes(uint32_t R) { return G::bytes() + param_bytes(R); }
+  // cell table holding only the first l2 level-2 tables
+  static __host__ __device__ size_t lut_bytes(uint32_t l2) { return ((size_t)G::L1CELLS + (size_t)l2 * 64) * sizeof(K); }
+  static __host__ __device__ size_t bytes(uint32_t R, uint32_t l2) { return lut_bytes(l2) + param_bytes(R); }
   // p must be 8-byte aligned
   __device__ void set_params(unsigned char* p, uint32_t R) {
     plo = reinterpret_cast<K*>(p);
@@ -467,9 +483,9 @@ template <class K> struct Pass0Map {
       pc = nullptr;
     }
   }
-  __device__ void carve(unsigned char* p, uint32_t R) {
+  __device__ void carve(unsigned char* p, uint32_t R, uint32_t l2 = G::L2TABLES) {
     lut = reinterpret_cast<K*>(p);
-    set_params(p + G::bytes(), R);
+    set_params(p + lut_bytes(l2), R);
   }
   // g: a Range<K> or a MapParam<K> (fields lo, mult, base, sh)
   template <class P> __device__ __forceinline__ void store_param(uint32_t r, const P& g) {
@@ -488,8 +504,9 @@ template <class K> struct Pass0Map {
   __device__ void load_params(const Range<K>* __restrict__ g, uint32_t R) {
     for (uint32_t i = threadIdx.x; i < R; i += blockDim.x) store_param(i, g[i]);
   }
-  __device__ void load(const K* __restrict__ g_lut, const Range<K>* __restrict__ g, uint32_t R) {
-    for (uint32_t i = threadIdx.x; i < G::L1CELLS + G::L2TABLES * 64; i += blockDim.x) lut[i] = g_lut[i];
+  __device__ void load(const K* __restrict__ g_lut, const Range<K>* __restrict__ g, uint32_t R,
+                       uint32_t l2 = G::L2TABLES) {
+    for (uint32_t i = threadIdx.x; i < G::L1CELLS + l2 * 64; i += blockDim.x) lut[i] = g_lut[i];
     load_params(g, R);
   }
   // bucket of key in range r (branch-free, equal to param_bucket)
@@ -587,9 +604,10 @@ template <class K> __device__ __forceinline__ uint32_t last_start_le(const K* st
 
 // Two-level table of a range list given its sorted starts (in LDS; the ranges
 // partition the key space, starts[0] == 0, R <= 2047).  Called by a whole
-// workgroup; writes level 1 and the level-2 tables to g_lut.
+// workgroup; writes level 1 and the level-2 tables to g_lut and returns the
+// number of level-2 tables in use (same value in every thread).
 template <class K>
-__device__ void build_lut(const K* starts, uint32_t R, K* __restrict__ g_lut) {
+__device__ uint32_t build_lut(const K* starts, uint32_t R, K* __restrict__ g_lut) {
   using G = LutGeom<K>;
   __shared__ uint32_t tab_cell[G::L2TABLES];
   __shared__ uint32_t scan_s[32];
@@ -639,6 +657,7 @@ __device__ void build_lut(const K* starts, uint32_t R, K* __restrict__ g_lut) {
     else e = G::FLAG | (K)r0 | ((K)r1 << 11);
     g_lut[G::L1CELLS + i] = e;
   }
+  return used;
 }
 
 // Plain fill (used instead of hipMemsetAsync so the sequence stays capturable).
@@ -825,6 +844,8 @@ template <class K> struct PlanArgs {
   uint32_t force_finish;     // nonzero: stop after this pass (all active buckets must fit fin_cap)
   uint32_t merge_small;      // nonzero: small buckets also become next-pass ranges (one bucket each)
   MapParam<K>* next_params;  // packed map parameters of next_ranges (or null)
+  uint64_t* rank_words;      // slot rank words of this pass's buckets (or null)
+  PassCtl* mirror;           // host-visible copy of ctl and ctl_next, written last (or null)
 };
 
 template <class T>
@@ -987,6 +1008,21 @@ __global__ void __launch_bounds__(1024) k_plan(PlanArgs<typename KeyTraits<T>::K
   __threadfence_block();
   __syncthreads();
 
+  // slot rank words (32 buckets each): has-a-slot bits, slot of the first
+  if (a.rank_words) {
+    for (uint32_t w = t; w < rank_word_count(B); w += nt) {
+      uint32_t bits = 0, first = 0;
+      for (uint32_t i = 0; i < 32 && w * 32 + i < B; ++i) {
+        uint32_t sv = a.slot_of_bucket[w * 32 + i];
+        if (sv != kNone16) {
+          if (!bits) first = sv;
+          bits |= 1u << i;
+        }
+      }
+      a.rank_words[w] = ((uint64_t)first << 32) | bits;
+    }
+  }
+
   // P6: bucket bases of the next pass
   uint32_t B_next = 0;
   {
@@ -1101,6 +1137,12 @@ __global__ void __launch_bounds__(1024) k_plan(PlanArgs<typename KeyTraits<T>::K
     if (ctl.nan_lo_bucket != kNone32) nan += a.totals[ctl.nan_lo_bucket];
     if (ctl.nan_hi_bucket != kNone32) nan += a.totals[ctl.nan_hi_bucket];
     cc->nan_count = nan;
+    // the host waits for this kernel, not for the compaction behind it: hand it
+    // both control blocks (read back by the thread that wrote them)
+    if (a.mirror) {
+      a.mirror[0] = *cc;
+      a.mirror[1] = *cn;
+    }
   }
 }
 
@@ -1219,24 +1261,34 @@ __global__ void __launch_bounds__(1024) k_compact0(const T* __restrict__ x, cons
 
 // Pass 1 reads the append-layout buffer: an element's range is the slot of its
 // first-pass bucket (recomputed from the key with the first-pass map), and the
-// range's own map gives the pass-1 bucket.  The pass-1 ranges' parameters are
-// staged in LDS, indexed by slot.
+// range's own map gives the pass-1 bucket.  The first pass numbers its slots in
+// bucket order, so a bucket's slot is a rank: one 64-bit word per 32 buckets
+// holds which of them have a slot (low half) and the slot of the first such
+// bucket (high half), slot = high + popcount(low & bits below).  With only the
+// level-2 tables the first pass used, the LDS footprint (cell table, first-pass
+// parameters, rank words, pass-1 parameters, histogram) leaves room for two
+// workgroups per CU.  Every element here comes from a slotted bucket.
 template <class K> struct Pass1Map {
   Pass0Map<K> m0;
-  const uint16_t* slot0;               // LDS: first-pass bucket -> slot (= pass-1 range)
+  const uint64_t* rank0;               // LDS: ceil(B0 / 32) rank words
   const MapParam<K>* p1;               // pass-1 range parameters by slot, global memory
   Pass0Map<K> p1s;                     // ... or as per-field LDS arrays (p1_lds)
   bool p1_lds;
+  __device__ __forceinline__ uint32_t slot0(uint32_t b0) const {
+    const uint64_t e = rank0[b0 >> 5];
+    const uint32_t below = (uint32_t)e & ((1u << (b0 & 31u)) - 1u);
+    return (uint32_t)(e >> 32) + (uint32_t)__builtin_popcount(below);
+  }
   __device__ __forceinline__ uint32_t range_bucket(uint32_t s0, K key) const {
     return p1_lds ? p1s.range_bucket(s0, key) : param_bucket<K>(p1[s0], key);
   }
-  __device__ __forceinline__ uint32_t bucket(K key) const { return range_bucket(slot0[m0.bucket(key)], key); }
+  __device__ __forceinline__ uint32_t bucket(K key) const { return range_bucket(slot0(m0.bucket(key)), key); }
   template <int N> __device__ __forceinline__ void buckets(const K (&key)[N], uint32_t (&b)[N]) const {
     uint32_t b0[N];
     m0.template buckets<N>(key, b0);
     uint32_t s0[N];
 #pragma unroll
-    for (int j = 0; j < N; ++j) s0[j] = slot0[b0[j]];
+    for (int j = 0; j < N; ++j) s0[j] = slot0(b0[j]);
     if (p1_lds) {
 #pragma unroll
       for (int j = 0; j < N; ++j) b[j] = p1s.range_bucket(s0[j], key[j]);
@@ -1249,21 +1301,21 @@ template <class K> struct Pass1Map {
 
 // LDS bytes of the pass-1 map; R1 = 0 when the parameters stay in global memory.
 template <class K>
-__host__ __device__ __forceinline__ size_t pass1_map_bytes(uint32_t B0, uint32_t R0, uint32_t R1) {
-  return Pass0Map<K>::bytes(R0) + ((size_t)B0 * 2 + 15) / 16 * 16 + Pass0Map<K>::param_bytes(R1);
+__host__ __device__ __forceinline__ size_t pass1_map_bytes(uint32_t B0, uint32_t R0, uint32_t l2, uint32_t R1) {
+  return Pass0Map<K>::bytes(R0, l2) + ((size_t)rank_word_count(B0) * 8 + 15) / 16 * 16 + Pass0Map<K>::param_bytes(R1);
 }
 
 template <class K>
 __device__ __forceinline__ void pass1_map_init(Pass1Map<K>& pm, unsigned char* p, const PassCtl* ctl0,
-                                               const K* lut, const Range<K>* rng0, const uint16_t* slot0_g,
+                                               const K* lut, const Range<K>* rng0, const uint64_t* rank0_g,
                                                const MapParam<K>* p1_g, uint32_t R1, bool p1_in_lds) {
-  const uint32_t B0 = ctl0->B, R0 = ctl0->R;
-  pm.m0.carve(p, R0);  // first-pass cell table first (p is LDS offset 0 in the pass-1 kernels)
-  pm.m0.load(lut, rng0, R0);
-  uint16_t* s0 = reinterpret_cast<uint16_t*>(p + Pass0Map<K>::bytes(R0));
-  for (uint32_t i = threadIdx.x; i < B0; i += blockDim.x) s0[i] = slot0_g[i];
-  pm.slot0 = s0;
-  unsigned char* q = p + Pass0Map<K>::bytes(R0) + ((size_t)B0 * 2 + 15) / 16 * 16;
+  const uint32_t B0 = ctl0->B, R0 = ctl0->R, l2 = ctl0->l2_used;
+  pm.m0.carve(p, R0, l2);  // first-pass cell table first (p is LDS offset 0 in the pass-1 kernels)
+  pm.m0.load(lut, rng0, R0, l2);
+  uint64_t* rk = reinterpret_cast<uint64_t*>(p + Pass0Map<K>::bytes(R0, l2));
+  for (uint32_t i = threadIdx.x; i < rank_word_count(B0); i += blockDim.x) rk[i] = rank0_g[i];
+  pm.rank0 = rk;
+  unsigned char* q = p + Pass0Map<K>::bytes(R0, l2) + ((size_t)rank_word_count(B0) * 8 + 15) / 16 * 16;
   pm.p1 = p1_g;
   pm.p1_lds = p1_in_lds;
   if (p1_in_lds) {
@@ -1275,16 +1327,16 @@ __device__ __forceinline__ void pass1_map_init(Pass1Map<K>& pm, unsigned char* p
 template <class K>
 __global__ void __launch_bounds__(1024) k_count1(const K* __restrict__ buf, const PassCtl* __restrict__ ctl0,
                                                  const K* __restrict__ lut, const Range<K>* __restrict__ rng0,
-                                                 const uint16_t* __restrict__ slot0, const MapParam<K>* __restrict__ p1,
+                                                 const uint64_t* __restrict__ rank0, const MapParam<K>* __restrict__ p1,
                                                  uint32_t p1_in_lds, const PassCtl* __restrict__ ctl,
                                                  uint32_t* __restrict__ rows) {
   extern __shared__ __align__(16) unsigned char smem[];
   const uint32_t B = ctl->B, n = ctl->n, tile = ctl->tile;
   if (blockIdx.x >= ctl->G) return;
   Pass1Map<K> pm;
-  pass1_map_init<K>(pm, smem, ctl0, lut, rng0, slot0, p1, ctl->R, p1_in_lds != 0);
+  pass1_map_init<K>(pm, smem, ctl0, lut, rng0, rank0, p1, ctl->R, p1_in_lds != 0);
   uint32_t* hist = reinterpret_cast<uint32_t*>(
-      smem + pass1_map_bytes<K>(ctl0->B, ctl0->R, p1_in_lds ? ctl->R : 0));
+      smem + pass1_map_bytes<K>(ctl0->B, ctl0->R, ctl0->l2_used, p1_in_lds ? ctl->R : 0));
   for (uint32_t i = threadIdx.x; i < B; i += blockDim.x) hist[i] = 0;
   __syncthreads();
   uint32_t t0 = blockIdx.x * tile;
@@ -1306,7 +1358,7 @@ __global__ void __launch_bounds__(1024) k_count1(const K* __restrict__ buf, cons
 template <class K>
 __global__ void __launch_bounds__(1024) k_compact1(const K* __restrict__ buf, const PassCtl* __restrict__ ctl0,
                                                    const K* __restrict__ lut, const Range<K>* __restrict__ rng0,
-                                                   const uint16_t* __restrict__ slot0, const MapParam<K>* __restrict__ p1,
+                                                   const uint64_t* __restrict__ rank0, const MapParam<K>* __restrict__ p1,
                                                    uint32_t p1_in_lds, const PassCtl* __restrict__ ctl,
                                                    const uint32_t* __restrict__ rows,
                                                    const uint16_t* __restrict__ slot_of_bucket,
@@ -1316,9 +1368,9 @@ __global__ void __launch_bounds__(1024) k_compact1(const K* __restrict__ buf, co
   const uint32_t B = ctl->B, n = ctl->n, tile = ctl->tile, U_slots = ctl->U_slots;
   if (blockIdx.x >= ctl->G || U_slots == 0) return;
   Pass1Map<K> pm;
-  pass1_map_init<K>(pm, smem, ctl0, lut, rng0, slot0, p1, ctl->R, p1_in_lds != 0);
+  pass1_map_init<K>(pm, smem, ctl0, lut, rng0, rank0, p1, ctl->R, p1_in_lds != 0);
   CompactTables ct;
-  compact_init(ct, smem + pass1_map_bytes<K>(ctl0->B, ctl0->R, p1_in_lds ? ctl->R : 0), B, U_slots, slot_of_bucket,
+  compact_init(ct, smem + pass1_map_bytes<K>(ctl0->B, ctl0->R, ctl0->l2_used, p1_in_lds ? ctl->R : 0), B, U_slots, slot_of_bucket,
                slot_dst, slot_bucket, rows + (size_t)blockIdx.x * B);
   __syncthreads();
   uint32_t t0 = blockIdx.x * tile;
@@ -1484,7 +1536,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8)
   for (uint32_t j = blockIdx.x; j < U_small; j += gridDim.x) {
     uint32_t st = small_start[j], len = small_len[j];
     if (len > fin_cap) {  // cannot happen unless the pass cap forced finishing; reported, never sorted
-      if (threadIdx.x == 0 && len_hi == fin_cap) atomicAdd(err, 1u);
+      if (threadIdx.x == 0 && len_hi == fin_cap) *err = 1u;  // a flag word in host-visible memory
       continue;
     }
     if (len <= len_lo || len > len_hi) continue;  // another launch's size class

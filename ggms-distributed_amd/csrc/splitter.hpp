@@ -338,7 +338,8 @@ __global__ void __launch_bounds__(1024) k_plan0(const T* __restrict__ x, Plan0Ar
     ctl->nan_hi_bucket = KT::kFloat ? Btot - 1 : kNone32;
   }
   // 5. key-prefix cell table, from the LDS copy of the range starts
-  build_lut<K>(rlo, R, lut);
+  const uint32_t l2 = build_lut<K>(rlo, R, lut);
+  if (threadIdx.x == 0) ctl->l2_used = l2;
 }
 
 }  // namespace ggms
