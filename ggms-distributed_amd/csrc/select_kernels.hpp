@@ -243,9 +243,9 @@ template <> struct Vec16<uint64_t> {
   }
 };
 
-// 16-byte streaming load with the nontemporal hint: each element is read once,
-// so the caches need not keep it (measured on one MI355X: a plain 1 GiB read
-// sweep 6.6 TB/s, with the hint 7.0 TB/s).
+// 16-byte streaming load with the nontemporal hint, for data each element of
+// which is read once (measured on one MI355X: a plain 1 GiB read sweep
+// 6.6 TB/s, with the hint 7.0 TB/s).
 __device__ __forceinline__ uint4 ld_stream(const uint4* p) {
   typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
   const u32x4 t = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(p));
@@ -263,10 +263,10 @@ __device__ __forceinline__ void stream_tile(const V* __restrict__ x, uint32_t t0
   uint32_t v1 = t1 / N;
   uint32_t v = t0 / N + threadIdx.x;
   for (; v + 3 * nt < v1; v += 4 * nt) {
-    uint4 q0 = ld_stream(xv + v);
-    uint4 q1 = ld_stream(xv + v + nt);
-    uint4 q2 = ld_stream(xv + v + 2 * nt);
-    uint4 q3 = ld_stream(xv + v + 3 * nt);
+    uint4 q0 = xv[v];
+    uint4 q1 = xv[v + nt];
+    uint4 q2 = xv[v + 2 * nt];
+    uint4 q3 = xv[v + 3 * nt];
     V e[4][N];
     Vec16<V>::unpack(q0, e[0]);
     Vec16<V>::unpack(q1, e[1]);
@@ -278,7 +278,7 @@ __device__ __forceinline__ void stream_tile(const V* __restrict__ x, uint32_t t0
       for (int j = 0; j < N; ++j) f(e[u][j], (v + u * nt) * N + j);
   }
   for (; v < v1; v += nt) {
-    uint4 q0 = ld_stream(xv + v);
+    uint4 q0 = xv[v];
     V e[N];
     Vec16<V>::unpack(q0, e);
 #pragma unroll
@@ -290,8 +290,12 @@ __device__ __forceinline__ void stream_tile(const V* __restrict__ x, uint32_t t0
 
 // Vector form: fv(const V (&e)[N]) receives the N values of each 16-byte load
 // (so per-element table lookups of one load can be issued together), f1(V)
-// the ragged tail.  Same alignment rules as stream_tile.
-template <class V, class FV, class F1>
+// the ragged tail.  Same alignment rules as stream_tile.  NT: nontemporal
+// loads, for data read from HBM exactly once (the caller's vector in the first
+// pass).  Later passes read what the previous compaction just wrote, much of
+// it still in the last-level cache, and lose with the hint (pass-1 compaction
+// 0.151 ms plain, 0.160 ms with it).
+template <class V, bool NT = false, class FV, class F1>
 __device__ __forceinline__ void stream_tile_vec(const V* __restrict__ x, uint32_t t0, uint32_t t1, FV&& fv, F1&& f1) {
   constexpr int N = Vec16<V>::N;
   const uint32_t nt = blockDim.x;
@@ -299,10 +303,10 @@ __device__ __forceinline__ void stream_tile_vec(const V* __restrict__ x, uint32_
   uint32_t v1 = t1 / N;
   uint32_t v = t0 / N + threadIdx.x;
   for (; v + 3 * nt < v1; v += 4 * nt) {
-    uint4 q0 = ld_stream(xv + v);
-    uint4 q1 = ld_stream(xv + v + nt);
-    uint4 q2 = ld_stream(xv + v + 2 * nt);
-    uint4 q3 = ld_stream(xv + v + 3 * nt);
+    uint4 q0 = NT ? ld_stream(xv + v) : xv[v];
+    uint4 q1 = NT ? ld_stream(xv + v + nt) : xv[v + nt];
+    uint4 q2 = NT ? ld_stream(xv + v + 2 * nt) : xv[v + 2 * nt];
+    uint4 q3 = NT ? ld_stream(xv + v + 3 * nt) : xv[v + 3 * nt];
     V e[N];
     Vec16<V>::unpack(q0, e);
     fv(e);
@@ -314,7 +318,7 @@ __device__ __forceinline__ void stream_tile_vec(const V* __restrict__ x, uint32_
     fv(e);
   }
   for (; v < v1; v += nt) {
-    uint4 q0 = ld_stream(xv + v);
+    uint4 q0 = NT ? ld_stream(xv + v) : xv[v];
     V e[N];
     Vec16<V>::unpack(q0, e);
     fv(e);
@@ -325,7 +329,7 @@ __device__ __forceinline__ void stream_tile_vec(const V* __restrict__ x, uint32_
 
 // Any start: the elements before the first 16-byte boundary go one per lane,
 // the rest through the vector path.
-template <class V, class FV, class F1>
+template <class V, bool NT = false, class FV, class F1>
 __device__ __forceinline__ void stream_tile_vec_any(const V* __restrict__ x, uint32_t t0, uint32_t t1, FV&& fv,
                                                     F1&& f1) {
   if (t0 >= t1) return;
@@ -337,7 +341,7 @@ __device__ __forceinline__ void stream_tile_vec_any(const V* __restrict__ x, uin
   uint32_t head = mis ? (16u - mis) / (uint32_t)sizeof(V) : 0u;
   if (head > t1 - t0) head = t1 - t0;
   if (threadIdx.x < head) f1(x[t0 + threadIdx.x]);
-  stream_tile_vec<V>(x + t0 + head, 0, t1 - t0 - head, fv, f1);
+  stream_tile_vec<V, NT>(x + t0 + head, 0, t1 - t0 - head, fv, f1);
 }
 
 // hist[b[j]] += 1 for N buckets per lane (every lane of the wave active for
@@ -708,7 +712,7 @@ __global__ void __launch_bounds__(1024) k_count0(const T* __restrict__ x, const 
   uint32_t t0 = blockIdx.x * tile;
   uint32_t t1 = t0 + tile < n ? t0 + tile : n;
   constexpr int N = Vec16<T>::N;
-  stream_tile_vec_any<T>(
+  stream_tile_vec_any<T, true>(
       x, t0, t1,
       [&](const T (&v)[N]) {
         K k[N];
@@ -1216,7 +1220,7 @@ __global__ void __launch_bounds__(1024) k_compact0(const T* __restrict__ x, cons
   uint32_t t0 = blockIdx.x * tile;
   uint32_t t1 = t0 + tile < n ? t0 + tile : n;
   constexpr int N = Vec16<T>::N;
-  stream_tile_vec_any<T>(
+  stream_tile_vec_any<T, true>(
       x, t0, t1,
       [&](const T (&v)[N]) {
         K k[N];
@@ -1429,13 +1433,18 @@ __global__ void __launch_bounds__(1024) k_compactS(const K* __restrict__ buf, co
 // k_finish: exact answers for the ks of every small segment.
 // A launch handles the segments of one size class (len_lo, len_hi].
 // The segment (<= fin_cap keys) is loaded into LDS once.  A segment holding
-// few ks answers each by radix select: 8-bit digits over only the low bits in
-// which the segment's key interval [klo, khi] varies (all keys share the
-// bits above), one LDS histogram + one wave scan per digit.  A segment holding
-// many ks is sorted instead (bitonic network in LDS) and read at every rank.
+// few ks answers all of them by one radix select: 8-bit digits over only the
+// low bits in which the segment's key interval [klo, khi] varies (all keys
+// share the bits above).  Each round builds one LDS histogram per distinct
+// prefix the ks have reached (ks sorted by rank, so equal prefixes are
+// neighbours) in a single sweep over the keys, and each k then takes its digit
+// from its prefix's histogram.  A segment holding many ks is sorted instead
+// (bitonic network in LDS) and read at every rank.
 // ---------------------------------------------------------------------------
 constexpr uint32_t kSelectMaxKs = 8;
 
+// Radix select of one rank (the batched kernel's finish): same digits, one
+// 256-bin histogram per round.
 template <class K>
 __device__ K radix_select_lds(const K* __restrict__ s, uint32_t len, K klo, K khi, uint32_t rank, uint32_t* hist,
                               uint32_t* bcast) {
@@ -1475,6 +1484,81 @@ __device__ K radix_select_lds(const K* __restrict__ s, uint32_t len, K klo, K kh
     __syncthreads();
   }
   return prefix;
+}
+
+template <class K> struct SelectLds {
+  uint32_t hist[kSelectMaxKs * 256];
+  uint32_t kid[kSelectMaxKs];   // k index, sorted by rank
+  uint32_t rank[kSelectMaxKs];  // 1-based rank inside the segment, then inside the prefix
+  uint32_t grp[kSelectMaxKs];   // histogram of the k's prefix this round
+  uint32_t ng;
+  K pre[kSelectMaxKs];          // digits found so far
+  K gpre[kSelectMaxKs];         // distinct prefixes of this round
+};
+
+// On entry kid/rank hold q (<= kSelectMaxKs) ks sorted by rank; on exit pre[j]
+// is the key of rank rank[j] among s[0, len).  Whole workgroup.
+template <class K>
+__device__ void radix_select_multi(const K* __restrict__ s, uint32_t len, K klo, K khi, uint32_t q, SelectLds<K>& L) {
+  constexpr uint32_t kb = sizeof(K) * 8;
+  const uint32_t vary = bitlen64((uint64_t)(klo ^ khi));
+  const K base = vary >= kb ? (K)0 : (K)(klo & (K)(~(K)0 << vary));
+  const uint32_t top = (vary + 7) & ~7u;
+  const uint32_t wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  if (threadIdx.x < q) L.pre[threadIdx.x] = base;
+  __syncthreads();
+  for (int shift = (int)top - 8; shift >= 0; shift -= 8) {
+    const K hmask = (uint32_t)shift + 8 >= kb ? (K)0 : (K)(~(K)0 << (shift + 8));
+    if (threadIdx.x == 0) {
+      uint32_t ng = 0;
+      for (uint32_t j = 0; j < q; ++j) {
+        const K pj = (K)(L.pre[j] & hmask);
+        if (ng == 0 || L.gpre[ng - 1] != pj) L.gpre[ng++] = pj;
+        L.grp[j] = ng - 1;
+      }
+      L.ng = ng;
+    }
+    __syncthreads();
+    const uint32_t ng = L.ng;
+    K gp[kSelectMaxKs];  // the round's prefixes in registers (entries >= ng unused)
+#pragma unroll
+    for (uint32_t i = 0; i < kSelectMaxKs; ++i) gp[i] = L.gpre[i];
+    for (uint32_t i = threadIdx.x; i < ng * 256; i += blockDim.x) L.hist[i] = 0;
+    __syncthreads();
+    if (ng == 1) {  // every k shares one prefix: the common case
+      for (uint32_t i = threadIdx.x; i < len; i += blockDim.x) {
+        const K key = s[i];
+        lds_count(L.hist, (uint32_t)(key >> shift) & 255u, (K)(key & hmask) == gp[0]);
+      }
+    } else {
+      for (uint32_t i = threadIdx.x; i < len; i += blockDim.x) {
+        const K key = s[i];
+        const K km = (K)(key & hmask);
+        uint32_t g = ng;  // the prefixes are distinct: at most one matches
+#pragma unroll
+        for (uint32_t p = 0; p < kSelectMaxKs; ++p)
+          if (p < ng && gp[p] == km) g = p;
+        lds_count(L.hist, g * 256 + ((uint32_t)(key >> shift) & 255u), g < ng);
+      }
+    }
+    __syncthreads();
+    for (uint32_t j = wv; j < q; j += nw) {
+      const uint32_t l = threadIdx.x & 63;
+      const uint32_t* h = L.hist + L.grp[j] * 256;
+      const uint32_t want = L.rank[j];
+      uint32_t c[4] = {h[4 * l], h[4 * l + 1], h[4 * l + 2], h[4 * l + 3]};
+      const uint32_t sum = c[0] + c[1] + c[2] + c[3];
+      uint32_t cum = wave_incl_scan_u32(sum) - sum;
+      for (uint32_t b = 0; b < 4; ++b) {
+        if (cum < want && cum + c[b] >= want) {
+          L.pre[j] = (K)(L.pre[j] | ((K)(4 * l + b) << shift));
+          L.rank[j] = want - cum;
+        }
+        cum += c[b];
+      }
+    }
+    __syncthreads();
+  }
 }
 
 // Loads buf[st, st + len) into s[0, len) with 16-byte loads, up to kSegLoadVecs
@@ -1529,8 +1613,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8)
                                                  uint32_t* __restrict__ err) {
   using K = typename KeyTraits<T>::K;
   extern __shared__ __align__(16) unsigned char smem[];
-  __shared__ uint32_t hist[256];
-  __shared__ uint32_t bcast[2];
+  __shared__ SelectLds<K> L;
   K* s = reinterpret_cast<K*>(smem);
   const uint32_t U_small = ctl->U_small;
   for (uint32_t j = blockIdx.x; j < U_small; j += gridDim.x) {
@@ -1542,20 +1625,38 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8)
     if (len <= len_lo || len > len_hi) continue;  // another launch's size class
     uint32_t q0 = small_kstart[j], q1 = small_kstart[j + 1];
     if (q1 - q0 <= kSelectMaxKs) {
+      const uint32_t q = q1 - q0;
       K klo = small_klo[j], khi = small_khi[j];
-      // the first k and its rank are fetched while the segment loads
-      uint32_t k0 = q1 > q0 ? klist[q0] : 0u;
-      uint32_t r0 = q1 > q0 ? k_rank[k0] : 0u;
+      // the ks and their ranks are fetched while the segment loads
+      uint32_t kk = 0, kr = 0;
+      if (threadIdx.x < q) {
+        kk = klist[q0 + threadIdx.x];
+        kr = k_rank[kk];
+      }
       load_segment_lds<K>(s, buf, st, len);
+      if (threadIdx.x < q) {
+        L.kid[threadIdx.x] = kk;
+        L.rank[threadIdx.x] = kr;
+      }
       __syncthreads();
-      for (uint32_t q = q0; q < q1; ++q) {
-        uint32_t k = q == q0 ? k0 : klist[q];
-        uint32_t r = q == q0 ? r0 : k_rank[k];
-        K key = radix_select_lds<K>(s, len, klo, khi, r, hist, bcast);
-        if (threadIdx.x == 0) {
-          out[k] = KeyTraits<T>::dec(key);
-          k_state[k] = kDone;
+      if (threadIdx.x == 0) {  // insertion sort by rank (q <= 8)
+        for (uint32_t a = 1; a < q; ++a) {
+          const uint32_t ka = L.kid[a], ra = L.rank[a];
+          uint32_t b = a;
+          for (; b > 0 && L.rank[b - 1] > ra; --b) {
+            L.kid[b] = L.kid[b - 1];
+            L.rank[b] = L.rank[b - 1];
+          }
+          L.kid[b] = ka;
+          L.rank[b] = ra;
         }
+      }
+      __syncthreads();
+      radix_select_multi<K>(s, len, klo, khi, q, L);
+      if (threadIdx.x < q) {
+        const uint32_t k = L.kid[threadIdx.x];
+        out[k] = KeyTraits<T>::dec(L.pre[threadIdx.x]);
+        k_state[k] = kDone;
       }
     } else {
       uint32_t M = pow2ceil_u32(len);

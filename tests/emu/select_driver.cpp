@@ -1,0 +1,71 @@
+// Emulated workgroup runs of the finish kernel's multi-rank radix select
+// (radix_select_multi) against std::sort: segments of 1..16384 keys inside a
+// random key interval, 1..8 ranks each (duplicates allowed), 32- and 64-bit
+// keys, narrow intervals full of duplicates and the full key range.
+#include "hip/hip_runtime.h"
+#include <algorithm>
+#include <random>
+#include "select_kernels.hpp"
+thread_local dim3 threadIdx;
+dim3 blockIdx, blockDim, gridDim;
+EmuBarrier g_bar;
+unsigned char* g_dyn_smem;
+struct WaveSync { std::mutex m; std::condition_variable cv; uint32_t arrived = 0, gen = 0; uint32_t v[64]; uint32_t out[64]; };
+static WaveSync g_w[16];
+uint32_t emu_wave_incl_scan(uint32_t v) {
+  uint32_t w = threadIdx.x >> 6, l = threadIdx.x & 63; WaveSync& s = g_w[w];
+  std::unique_lock<std::mutex> lk(s.m); uint32_t g = s.gen; s.v[l] = v;
+  if (++s.arrived == 64) { uint32_t acc = 0; for (int i = 0; i < 64; ++i) { acc += s.v[i]; s.out[i] = acc; } s.arrived = 0; s.gen++; s.cv.notify_all(); }
+  else s.cv.wait(lk, [&] { return s.gen != g; });
+  return s.out[l];
+}
+template <class K> int run(uint64_t seed, uint32_t nt) {
+  std::mt19937_64 rng(seed);
+  const uint32_t len = 1 + (uint32_t)(rng() % 16384);
+  const int kind = (int)(rng() % 4);
+  K klo, khi;
+  if (kind == 0) { klo = 0; khi = (K)~(K)0; }                                   // full range
+  else {
+    const uint32_t bits = kind == 1 ? 3 : (kind == 2 ? 12 : 1 + (uint32_t)(rng() % (sizeof(K) * 8 - 1)));
+    const K w = (K)(((K)1 << bits) - 1);
+    klo = (K)rng();
+    if ((K)(klo + w) < klo) klo = (K)(klo - w);                                 // keep [klo, klo + w] unwrapped
+    khi = (K)(klo + (K)(rng() % ((uint64_t)w + 1)));
+  }
+  std::vector<K> a(len);
+  const uint64_t span = (uint64_t)(khi - klo);
+  for (auto& v : a) v = (K)(klo + (K)(span == ~0ull ? rng() : rng() % (span + 1)));
+  if (len > 1) a[rng() % len] = klo, a[rng() % len] = khi;  // the interval's ends may or may not occur
+  std::vector<K> sorted = a;
+  std::sort(sorted.begin(), sorted.end());
+  const uint32_t q = 1 + (uint32_t)(rng() % ggms::kSelectMaxKs);
+  static ggms::SelectLds<K> L;
+  std::vector<uint32_t> ranks(q);
+  for (auto& r : ranks) r = 1 + (uint32_t)(rng() % len);
+  if (q > 1 && (rng() & 1)) ranks[1] = ranks[0];  // a repeated rank
+  std::sort(ranks.begin(), ranks.end());
+  for (uint32_t j = 0; j < q; ++j) { L.kid[j] = j; L.rank[j] = ranks[j]; }
+  blockDim = dim3(nt); g_bar.reset(nt);
+  std::vector<std::thread> th;
+  for (uint32_t t = 0; t < nt; ++t)
+    th.emplace_back([&, t] { threadIdx = dim3(t); ggms::radix_select_multi<K>(a.data(), len, klo, khi, q, L); g_bar.exit_thread(); });
+  for (auto& x : th) x.join();
+  int bad = 0;
+  for (uint32_t j = 0; j < q; ++j)
+    if (L.pre[j] != sorted[ranks[j] - 1]) {
+      if (bad < 3) printf("FAIL K=%zu len=%u q=%u kind=%d rank=%u got %llx want %llx\n", sizeof(K), len, q, kind, ranks[j],
+                          (unsigned long long)L.pre[j], (unsigned long long)sorted[ranks[j] - 1]);
+      ++bad;
+    }
+  return bad ? 1 : 0;
+}
+int main(int argc, char** argv) {
+  const int trials = argc > 1 ? atoi(argv[1]) : 60;
+  int bad = 0;
+  for (int i = 0; i < trials; ++i) {
+    bad += run<uint32_t>(1000 + i, (i & 1) ? 256 : 128);
+    bad += run<uint64_t>(5000 + i, (i & 1) ? 128 : 256);
+  }
+  printf(bad ? "select FAIL\n" : "select ok\n");
+  return bad;
+}

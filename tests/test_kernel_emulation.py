@@ -3,7 +3,8 @@ with g++ against a small HIP shim (tests/emu/hip/hip_runtime.h) and run with
 one thread per work-item, checked barriers and one-lane waves.  This covers
 kernel logic that the GPU parity tests cannot isolate: barrier agreement on
 every path (including the error paths), the batched kernel's planning and
-work stack, the LDS sort network and the first-pass splitter's range table.
+work stack, the LDS sort network, the finish's multi-rank radix select and the
+first-pass splitter's range table.
 
 Each driver compares its results with std::sort / a binary-search map."""
 import os
@@ -19,9 +20,10 @@ EMU = os.path.join(ROOT, "tests", "emu")
 
 
 def _prepare(dst):
-    """Copy the kernel headers with the two substitutions the emulation needs:
-    the dynamic-LDS declaration becomes a pointer the driver sets, and the
-    wave-wide inclusive scan calls the emulated 64-thread scan."""
+    """Copy the kernel headers with the substitutions the emulation needs: the
+    dynamic-LDS declaration becomes a pointer the driver sets, the wave-wide
+    inclusive scan calls the emulated 64-thread scan, and the nontemporal
+    streaming load (a clang vector builtin) becomes a plain load."""
     for f in ["ggms_device.hpp", "philox.hpp", "ggms_internal.hpp", "select_kernels.hpp", "splitter.hpp"]:
         s = open(os.path.join(CSRC, f)).read()
         s = s.replace("extern __shared__ __align__(16) unsigned char smem[];", "unsigned char* smem = g_dyn_smem;")
@@ -29,6 +31,9 @@ def _prepare(dst):
             s, n = re.subn(r"__device__ __forceinline__ uint32_t wave_incl_scan_u32\(uint32_t v\) \{.*?\n\}\n",
                            "__device__ __forceinline__ uint32_t wave_incl_scan_u32(uint32_t v) "
                            "{ return emu_wave_incl_scan(v); }\n", s, flags=re.S)
+            assert n == 1
+            s, n = re.subn(r"__device__ __forceinline__ uint4 ld_stream\(const uint4\* p\) \{.*?\n\}\n",
+                           "__device__ __forceinline__ uint4 ld_stream(const uint4* p) { return *p; }\n", s, flags=re.S)
             assert n == 1
         open(os.path.join(dst, f), "w").write(s)
     s = open(os.path.join(CSRC, "batch.hip")).read()
@@ -61,6 +66,12 @@ def _build_and_run(gen_dir, driver, args=(), timeout=900):
 def test_lds_bitonic_sort_network(gen_dir):
     out = _build_and_run(gen_dir, "sort_driver.cpp")
     assert "sort ok" in out
+
+
+def test_finish_multi_rank_radix_select(gen_dir):
+    """Up to eight ranks of one segment in a single radix sweep per digit."""
+    out = _build_and_run(gen_dir, "select_driver.cpp", [60])
+    assert "select ok" in out
 
 
 def test_first_pass_splitter_ranges_and_cell_table(gen_dir):
