@@ -222,7 +222,7 @@ struct Prof {
 enum Buf {
   bBufA, bBufB, bRows, bTotals, bRanges0, bRanges1, bRangesFirst, bCtl, bKs, bKRank, bKState, bKAux, bKList,
   bSlotOfBucket, bSlotDst, bSlotBucket, bActBucket, bActCls, bActDst, bActSlot, bActKlo, bActKhi,
-  bSmallStart, bSmallLen, bSmallKStart, bSmallKFill, bWgFirst, bOut, bLut, bSmallKlo, bSmallKhi, bParams1, bActCnt, bRank0, bCount
+  bSmallStart, bSmallLen, bSmallKStart, bSmallKFill, bWgFirst, bOut, bLut, bSmallKlo, bSmallKhi, bParams1, bActCnt, bRank0, bSmallOrder, bCount
 };
 static_assert(bCount <= kMaxBuffers, "workspace layout table too small");
 
@@ -262,6 +262,7 @@ static Layout make_layout(uint64_t n, uint32_t nk, const Config& c) {
   sz[bSmallLen] = Bm * 4;
   sz[bSmallKStart] = (Bm + 1) * 4;
   sz[bSmallKFill] = Bm * 4;
+  sz[bSmallOrder] = Bm * 4;
   sz[bWgFirst] = (size_t)c.G_max * 4 * 2;  // double-buffered by pass parity
   sz[bOut] = nkp * 8;
   sz[bLut] = LutGeom<K>::bytes();
@@ -389,8 +390,9 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
   HIP_TRY(hipGetLastError());
 
   const uint32_t fin_grid = std::max<uint32_t>(1, std::min<uint32_t>(4 * (uint32_t)ctx->cus, 65535));
-  const size_t fin_lds = (size_t)c.fin_cap * sizeof(K);
-  set_lds_attr((const void*)k_finish<T>, fin_lds);
+  const size_t fin_lds_half = (size_t)(c.fin_cap / 2) * sizeof(K);  // two workgroups per CU
+  const size_t fin_lds_full = (size_t)c.fin_cap * sizeof(K);
+  set_lds_attr((const void*)k_finish<T>, fin_lds_full);
 
   if (n <= c.fin_cap) {
     prof.begin(GGMS_KERNEL_OTHER, -1);
@@ -400,9 +402,12 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
     prof.end();
     HIP_TRY(hipGetLastError());
     prof.begin(GGMS_KERNEL_FINISH, 0);
-    k_finish<T><<<1, 1024, fin_lds, st>>>(bufA, ctl, (uint32_t*)P(bSmallStart), (uint32_t*)P(bSmallLen),
-                                          (K*)P(bSmallKlo), (K*)P(bSmallKhi), (uint32_t*)P(bSmallKStart), klist, k_rank, k_state, d_out,
-                                          0u, c.fin_cap, c.fin_cap, dflag);
+    // one segment over the whole key range: the half layout holds it as keys
+    // when it has at most fin_cap / 2 of them
+    const uint32_t full = n > c.fin_cap / 2 ? 1u : 0u;
+    k_finish<T><<<1, 1024, full ? fin_lds_full : fin_lds_half, st>>>(
+        bufA, ctl, (uint32_t*)P(bSmallStart), (uint32_t*)P(bSmallLen), (K*)P(bSmallKlo), (K*)P(bSmallKhi),
+        (uint32_t*)P(bSmallKStart), nullptr, klist, k_rank, k_state, d_out, full, c.fin_cap, dflag);
     prof.end();
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(&hctl[0], ctl, ctlb, hipMemcpyDeviceToHost, st));
@@ -511,6 +516,7 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
       pl.small_khi = (K*)P(bSmallKhi);
       pl.small_kstart = (uint32_t*)P(bSmallKStart);
       pl.small_kfill = (uint32_t*)P(bSmallKFill);
+      pl.small_order = (uint32_t*)P(bSmallOrder);
       pl.klist = klist;
       pl.wg_first_next = wgf_next;
       pl.out = d_out;
@@ -565,19 +571,22 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
         prof.end();
       }
       HIP_TRY(hipGetLastError());
-      if (pass > 0) {  // the first pass hands every k-bucket to pass 1 (no small segments yet)
-      prof.begin(GGMS_KERNEL_FINISH, (int)pass);
-      k_finish<T><<<fin_grid, 1024, fin_lds, st>>>(dst, cp, pl.small_start, pl.small_len, pl.small_klo,
-                                                   pl.small_khi, pl.small_kstart, klist, k_rank, k_state, d_out,
-                                                   0u, c.fin_cap, c.fin_cap, dflag);
-      prof.end();
-      HIP_TRY(hipGetLastError());
-      }
-      // the next pass is planned from the control blocks the plan kernel wrote
-      // to host memory, while this pass's compaction and finish still run
+      // the next pass (and this pass's finish) is planned from the control
+      // blocks the plan kernel wrote to host memory, while the compaction runs
       HIP_TRY(hipEventSynchronize(ctx->plan_ev));
       const PassCtl h = hmirror[0];
       const PassCtl hn = hmirror[1];
+      if (pass > 0 && h.U_small > 0) {  // the first pass hands every k-bucket to pass 1
+        for (uint32_t full = 0; full < 2; ++full) {
+          if (full && h.n_full == 0) break;
+          prof.begin(GGMS_KERNEL_FINISH, (int)pass);
+          k_finish<T><<<fin_grid, 1024, full ? fin_lds_full : fin_lds_half, st>>>(
+              dst, cp, pl.small_start, pl.small_len, pl.small_klo, pl.small_khi, pl.small_kstart, pl.small_order,
+              klist, k_rank, k_state, d_out, full, c.fin_cap, dflag);
+          prof.end();
+          HIP_TRY(hipGetLastError());
+        }
+      }
       if (pass == 0 && (h.bad_k || h.nan_count)) {
         HIP_TRY(hipStreamSynchronize(st));
         return h.bad_k ? GGMS_EINVAL : GGMS_ENAN;

@@ -59,7 +59,8 @@ struct PassCtl {
   uint32_t bad_k;          // number of ks outside [1, n]
   uint32_t append;         // first-pass compaction: next free position of the survivor buffer
   uint32_t l2_used;        // first pass: level-2 tables of the cell table in use
-  uint32_t pad[13];
+  uint32_t n_full;         // small segments that need the finish's full LDS layout
+  uint32_t pad[12];
 };
 
 struct Plan0Args {
@@ -185,10 +186,12 @@ __device__ void bitonic_sort_lds_reg(K* s, uint32_t M) {
 template <class K>
 __device__ void bitonic_sort_lds(K* s, uint32_t M) {
   const uint32_t e = M / blockDim.x;
-  if (M == e * blockDim.x && (e == 4 || e == 8 || (e == 16 && sizeof(K) == 4))) {
+  if (M == e * blockDim.x &&
+      (e == 4 || e == 8 || (e == 16 && sizeof(K) <= 4) || (e == 32 && sizeof(K) == 2))) {
     if (e == 4) bitonic_sort_lds_reg<K, 4>(s, M);
     else if (e == 8) bitonic_sort_lds_reg<K, 8>(s, M);
-    else bitonic_sort_lds_reg<K, 16>(s, M);
+    else if (e == 16) bitonic_sort_lds_reg<K, 16>(s, M);
+    else bitonic_sort_lds_reg<K, 32>(s, M);
     return;
   }
   for (uint32_t size = 2; size <= M; size <<= 1) {
@@ -198,6 +201,26 @@ __device__ void bitonic_sort_lds(K* s, uint32_t M) {
     }
   }
 }
+
+// Half-width word of a key type: offsets from a segment's low key that fit
+// one are stored in it.
+template <class K> struct HalfKey;
+template <> struct HalfKey<uint32_t> { using T = uint16_t; };
+template <> struct HalfKey<uint64_t> { using T = uint32_t; };
+
+// The finish runs in two layouts.  Half: fin_cap / 2 full-width keys of LDS,
+// which holds a segment of at most fin_cap / 2 keys as keys, or one of at most
+// fin_cap keys whose interval is narrow enough for half-width offsets (two
+// workgroups per CU).  Full: fin_cap keys, for the rest (one per CU).
+template <class K>
+__host__ __device__ __forceinline__ bool finish_fits_half(uint32_t len, K klo, K khi, uint32_t fin_cap) {
+  using H = typename HalfKey<K>::T;
+  return len <= fin_cap / 2 || (uint64_t)(K)(khi - klo) <= (uint64_t)(H)~(H)0;
+}
+
+// Size classes of small segments for the finish order (1024 keys per class).
+constexpr uint32_t kOrderBins = 33;
+__host__ __device__ __forceinline__ uint32_t order_bin(uint32_t len) { return len >> 10 < kOrderBins - 1 ? len >> 10 : kOrderBins - 1; }
 
 // words of a slot rank table (32 buckets per 64-bit word)
 __host__ __device__ __forceinline__ uint32_t rank_word_count(uint32_t B) { return (B + 31) / 32; }
@@ -837,6 +860,7 @@ template <class K> struct PlanArgs {
   K* small_khi;
   uint32_t* small_kstart;    // [U_small + 1]
   uint32_t* small_kfill;     // [U_small]
+  uint32_t* small_order;     // [U_small] segments in finish order
   uint32_t* klist;           // [nk]
   uint32_t* wg_first_next;   // [G_max]
   void* out;                 // T[nk]
@@ -864,6 +888,9 @@ __global__ void __launch_bounds__(1024) k_plan(PlanArgs<typename KeyTraits<T>::K
   uint32_t* P = reinterpret_cast<uint32_t*>(smem);     // inclusive prefix of the histogram
   uint32_t* aidx = P + B;                              // active flag -> index + 1
 
+  if (t == 0) sh_tot[5] = 0;  // small segments for the full finish layout
+  __shared__ uint32_t ord_cnt[kOrderBins];  // small segments per size class (finish order)
+  if (t < kOrderBins) ord_cnt[t] = 0;
   // P0: histogram prefix, reset tables
   for (uint32_t i = t; i < B; i += nt) {
     P[i] = a.totals[i];
@@ -1002,11 +1029,13 @@ __global__ void __launch_bounds__(1024) k_plan(PlanArgs<typename KeyTraits<T>::K
       a.next_ranges[s] = nr;
     } else {
       uint32_t j = s - U_big;
+      const K klo = a.act_klo[q], khi = a.act_khi[q];
       a.small_start[j] = dst;
       a.small_len[j] = c;
-      a.small_klo[j] = a.act_klo[q];
-      a.small_khi[j] = a.act_khi[q];
+      a.small_klo[j] = klo;
+      a.small_khi[j] = khi;
       a.small_kfill[j] = 0;
+      if (!finish_fits_half<K>(c, klo, khi, a.fin_cap)) atomicAdd(&sh_tot[5], 1u);
     }
   }
   __threadfence_block();
@@ -1077,7 +1106,10 @@ __global__ void __launch_bounds__(1024) k_plan(PlanArgs<typename KeyTraits<T>::K
     uint32_t per = (U_small + nt - 1) / nt;
     uint32_t j0 = t * per, j1 = j0 + per < U_small ? j0 + per : U_small;
     uint32_t s = 0;
-    for (uint32_t j = j0; j < j1; ++j) s += a.small_kfill[j];
+    for (uint32_t j = j0; j < j1; ++j) {
+      s += a.small_kfill[j];
+      atomicAdd(&ord_cnt[order_bin(a.small_len[j])], 1u);
+    }
     uint32_t tot;
     uint32_t off = block_excl_scan_u32(s, scratch, &tot);
     for (uint32_t j = j0; j < j1; ++j) {
@@ -1089,6 +1121,14 @@ __global__ void __launch_bounds__(1024) k_plan(PlanArgs<typename KeyTraits<T>::K
   }
   __threadfence_block();
   __syncthreads();
+  if (t == 0) {  // largest size class first
+    uint32_t acc = 0;
+    for (int c = (int)kOrderBins - 1; c >= 0; --c) {
+      const uint32_t v = ord_cnt[c];
+      ord_cnt[c] = acc;
+      acc += v;
+    }
+  }
   uint32_t live = 0;
   for (uint32_t k = t; k < a.nk; k += nt) {
     uint32_t st = a.k_state[k];
@@ -1101,7 +1141,11 @@ __global__ void __launch_bounds__(1024) k_plan(PlanArgs<typename KeyTraits<T>::K
     if (st == kLive) live++;
   }
   uint32_t live_tot;
-  block_excl_scan_u32(live, scratch, &live_tot);
+  block_excl_scan_u32(live, scratch, &live_tot);  // (its barriers also publish ord_cnt)
+  // finish order of the small segments: larger ones first, so the longest
+  // workgroups start early instead of trailing the launch
+  for (uint32_t j = t; j < U_small; j += nt)
+    a.small_order[atomicAdd(&ord_cnt[order_bin(a.small_len[j])], 1u)] = j;
 
   // P9: next-pass tiling and first range of every tile
   uint32_t Gn = 0, tile = 0;
@@ -1137,6 +1181,7 @@ __global__ void __launch_bounds__(1024) k_plan(PlanArgs<typename KeyTraits<T>::K
     cc->N_big = N_big;
     cc->N_small = N_small;
     cc->live_ks = live_tot;
+    cc->n_full = sh_tot[5];
     uint32_t nan = 0;
     if (ctl.nan_lo_bucket != kNone32) nan += a.totals[ctl.nan_lo_bucket];
     if (ctl.nan_hi_bucket != kNone32) nan += a.totals[ctl.nan_hi_bucket];
@@ -1486,20 +1531,20 @@ __device__ K radix_select_lds(const K* __restrict__ s, uint32_t len, K klo, K kh
   return prefix;
 }
 
-template <class K> struct SelectLds {
+struct SelectLds {
   uint32_t hist[kSelectMaxKs * 256];
   uint32_t kid[kSelectMaxKs];   // k index, sorted by rank
   uint32_t rank[kSelectMaxKs];  // 1-based rank inside the segment, then inside the prefix
   uint32_t grp[kSelectMaxKs];   // histogram of the k's prefix this round
   uint32_t ng;
-  K pre[kSelectMaxKs];          // digits found so far
-  K gpre[kSelectMaxKs];         // distinct prefixes of this round
+  uint64_t pre[kSelectMaxKs];   // digits found so far (as the key type, widened)
+  uint64_t gpre[kSelectMaxKs];  // distinct prefixes of this round
 };
 
 // On entry kid/rank hold q (<= kSelectMaxKs) ks sorted by rank; on exit pre[j]
 // is the key of rank rank[j] among s[0, len).  Whole workgroup.
 template <class K>
-__device__ void radix_select_multi(const K* __restrict__ s, uint32_t len, K klo, K khi, uint32_t q, SelectLds<K>& L) {
+__device__ void radix_select_multi(const K* __restrict__ s, uint32_t len, K klo, K khi, uint32_t q, SelectLds& L) {
   constexpr uint32_t kb = sizeof(K) * 8;
   const uint32_t vary = bitlen64((uint64_t)(klo ^ khi));
   const K base = vary >= kb ? (K)0 : (K)(klo & (K)(~(K)0 << vary));
@@ -1512,8 +1557,8 @@ __device__ void radix_select_multi(const K* __restrict__ s, uint32_t len, K klo,
     if (threadIdx.x == 0) {
       uint32_t ng = 0;
       for (uint32_t j = 0; j < q; ++j) {
-        const K pj = (K)(L.pre[j] & hmask);
-        if (ng == 0 || L.gpre[ng - 1] != pj) L.gpre[ng++] = pj;
+        const K pj = (K)((K)L.pre[j] & hmask);
+        if (ng == 0 || (K)L.gpre[ng - 1] != pj) L.gpre[ng++] = pj;
         L.grp[j] = ng - 1;
       }
       L.ng = ng;
@@ -1522,7 +1567,7 @@ __device__ void radix_select_multi(const K* __restrict__ s, uint32_t len, K klo,
     const uint32_t ng = L.ng;
     K gp[kSelectMaxKs];  // the round's prefixes in registers (entries >= ng unused)
 #pragma unroll
-    for (uint32_t i = 0; i < kSelectMaxKs; ++i) gp[i] = L.gpre[i];
+    for (uint32_t i = 0; i < kSelectMaxKs; ++i) gp[i] = (K)L.gpre[i];
     for (uint32_t i = threadIdx.x; i < ng * 256; i += blockDim.x) L.hist[i] = 0;
     __syncthreads();
     if (ng == 1) {  // every k shares one prefix: the common case
@@ -1551,7 +1596,7 @@ __device__ void radix_select_multi(const K* __restrict__ s, uint32_t len, K klo,
       uint32_t cum = wave_incl_scan_u32(sum) - sum;
       for (uint32_t b = 0; b < 4; ++b) {
         if (cum < want && cum + c[b] >= want) {
-          L.pre[j] = (K)(L.pre[j] | ((K)(4 * l + b) << shift));
+          L.pre[j] = (K)((K)L.pre[j] | ((K)(4 * l + b) << shift));
           L.rank[j] = want - cum;
         }
         cum += c[b];
@@ -1562,14 +1607,16 @@ __device__ void radix_select_multi(const K* __restrict__ s, uint32_t len, K klo,
 }
 
 // Loads buf[st, st + len) into s[0, len) with 16-byte loads, up to kSegLoadVecs
-// per lane in flight (a segment of the finish capacity is one round trip).
+// per lane in flight (a segment of the finish capacity is one round trip),
+// storing key - base as type C (the key itself when C is K and base is 0).
 // The vectors are aligned down/up to 16 bytes; the keys outside the segment
 // they carry are dropped.  Every buffer this reads is allocated in 256-byte
 // multiples from a 256-byte aligned base, so the rounded vectors stay inside it.
 constexpr int kSegLoadVecs = 4;
 
-template <class K>
-__device__ __forceinline__ void load_segment_lds(K* __restrict__ s, const K* __restrict__ buf, uint32_t st, uint32_t len) {
+template <class K, class C>
+__device__ __forceinline__ void load_segment_lds(C* __restrict__ s, const K* __restrict__ buf, uint32_t st, uint32_t len,
+                                                 K base) {
   constexpr uint32_t N = 16 / sizeof(K);
   const uint32_t head = st & (N - 1);
   const uint4* src = reinterpret_cast<const uint4*>(buf + (st - head));
@@ -1580,7 +1627,7 @@ __device__ __forceinline__ void load_segment_lds(K* __restrict__ s, const K* __r
 #pragma unroll
     for (uint32_t q = 0; q < N; ++q) {
       uint32_t idx = i * N + q - head;  // wraps above len for the dropped head keys
-      if (idx < len) s[idx] = e[q];
+      if (idx < len) s[idx] = (C)(e[q] - base);
     }
   };
   uint4 v[kSegLoadVecs];
@@ -1597,9 +1644,59 @@ __device__ __forceinline__ void load_segment_lds(K* __restrict__ s, const K* __r
   for (uint32_t i = threadIdx.x + kSegLoadVecs * blockDim.x; i < nv; i += blockDim.x) put(i, src[i]);
 }
 
-// Eight waves per SIMD (two workgroups per CU, the LDS allows it) caps the
+// Answers the ks [q0, q1) of one segment held in LDS as type C, C value = key - base.
+template <class T, class C>
+__device__ __forceinline__ void finish_segment(C* __restrict__ s, uint32_t len, C clo, C chi,
+                                               typename KeyTraits<T>::K base, uint32_t q0, uint32_t q1,
+                                               uint32_t kk, uint32_t kr, const uint32_t* __restrict__ klist,
+                                               uint32_t* __restrict__ k_rank, uint32_t* __restrict__ k_state,
+                                               T* __restrict__ out, SelectLds& L) {
+  using K = typename KeyTraits<T>::K;
+  const uint32_t q = q1 - q0;
+  if (q <= kSelectMaxKs) {
+    if (threadIdx.x < q) {
+      L.kid[threadIdx.x] = kk;
+      L.rank[threadIdx.x] = kr;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {  // insertion sort by rank (q <= 8)
+      for (uint32_t a = 1; a < q; ++a) {
+        const uint32_t ka = L.kid[a], ra = L.rank[a];
+        uint32_t b = a;
+        for (; b > 0 && L.rank[b - 1] > ra; --b) {
+          L.kid[b] = L.kid[b - 1];
+          L.rank[b] = L.rank[b - 1];
+        }
+        L.kid[b] = ka;
+        L.rank[b] = ra;
+      }
+    }
+    __syncthreads();
+    radix_select_multi<C>(s, len, clo, chi, q, L);
+    if (threadIdx.x < q) {
+      const uint32_t k = L.kid[threadIdx.x];
+      out[k] = KeyTraits<T>::dec((K)(base + (K)(C)L.pre[threadIdx.x]));
+      k_state[k] = kDone;
+    }
+  } else {
+    const uint32_t M = pow2ceil_u32(len);
+    for (uint32_t i = len + threadIdx.x; i < M; i += blockDim.x) s[i] = (C)~(C)0;
+    __syncthreads();
+    bitonic_sort_lds<C>(s, M);
+    for (uint32_t qq = q0 + threadIdx.x; qq < q1; qq += blockDim.x) {
+      const uint32_t k = klist[qq];
+      out[k] = KeyTraits<T>::dec((K)(base + (K)s[k_rank[k] - 1]));
+      k_state[k] = kDone;
+    }
+  }
+}
+
+// Eight waves per SIMD (two workgroups per CU in the half layout) caps the
 // scalar registers: left alone the compiler takes 91 SGPRs, which admits only
 // one 1024-thread workgroup per CU.
+// full == 0: the half layout (dynamic LDS fin_cap / 2 keys) takes every segment
+// finish_fits_half accepts; full != 0: the full layout (fin_cap keys) takes the
+// others.
 template <class T>
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8))) k_finish(const typename KeyTraits<T>::K* __restrict__ buf, const PassCtl* __restrict__ ctl,
                                                  const uint32_t* __restrict__ small_start,
@@ -1607,68 +1704,40 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8)
                                                  const typename KeyTraits<T>::K* __restrict__ small_klo,
                                                  const typename KeyTraits<T>::K* __restrict__ small_khi,
                                                  const uint32_t* __restrict__ small_kstart,
+                                                 const uint32_t* __restrict__ small_order,
                                                  const uint32_t* __restrict__ klist, uint32_t* __restrict__ k_rank,
                                                  uint32_t* __restrict__ k_state, T* __restrict__ out,
-                                                 uint32_t len_lo, uint32_t len_hi, uint32_t fin_cap,
-                                                 uint32_t* __restrict__ err) {
+                                                 uint32_t full, uint32_t fin_cap, uint32_t* __restrict__ err) {
   using K = typename KeyTraits<T>::K;
+  using H = typename HalfKey<K>::T;
   extern __shared__ __align__(16) unsigned char smem[];
-  __shared__ SelectLds<K> L;
-  K* s = reinterpret_cast<K*>(smem);
+  __shared__ SelectLds L;
   const uint32_t U_small = ctl->U_small;
-  for (uint32_t j = blockIdx.x; j < U_small; j += gridDim.x) {
-    uint32_t st = small_start[j], len = small_len[j];
+  for (uint32_t jj = blockIdx.x; jj < U_small; jj += gridDim.x) {
+    const uint32_t j = small_order ? small_order[jj] : jj;
+    const uint32_t st = small_start[j], len = small_len[j];
     if (len > fin_cap) {  // cannot happen unless the pass cap forced finishing; reported, never sorted
-      if (threadIdx.x == 0 && len_hi == fin_cap) *err = 1u;  // a flag word in host-visible memory
+      if (threadIdx.x == 0) *err = 1u;  // a flag word in host-visible memory
       continue;
     }
-    if (len <= len_lo || len > len_hi) continue;  // another launch's size class
-    uint32_t q0 = small_kstart[j], q1 = small_kstart[j + 1];
-    if (q1 - q0 <= kSelectMaxKs) {
-      const uint32_t q = q1 - q0;
-      K klo = small_klo[j], khi = small_khi[j];
-      // the ks and their ranks are fetched while the segment loads
-      uint32_t kk = 0, kr = 0;
-      if (threadIdx.x < q) {
-        kk = klist[q0 + threadIdx.x];
-        kr = k_rank[kk];
-      }
-      load_segment_lds<K>(s, buf, st, len);
-      if (threadIdx.x < q) {
-        L.kid[threadIdx.x] = kk;
-        L.rank[threadIdx.x] = kr;
-      }
-      __syncthreads();
-      if (threadIdx.x == 0) {  // insertion sort by rank (q <= 8)
-        for (uint32_t a = 1; a < q; ++a) {
-          const uint32_t ka = L.kid[a], ra = L.rank[a];
-          uint32_t b = a;
-          for (; b > 0 && L.rank[b - 1] > ra; --b) {
-            L.kid[b] = L.kid[b - 1];
-            L.rank[b] = L.rank[b - 1];
-          }
-          L.kid[b] = ka;
-          L.rank[b] = ra;
-        }
-      }
-      __syncthreads();
-      radix_select_multi<K>(s, len, klo, khi, q, L);
-      if (threadIdx.x < q) {
-        const uint32_t k = L.kid[threadIdx.x];
-        out[k] = KeyTraits<T>::dec(L.pre[threadIdx.x]);
-        k_state[k] = kDone;
-      }
+    const K klo = small_klo[j], khi = small_khi[j];
+    const bool half = finish_fits_half<K>(len, klo, khi, fin_cap);
+    if (full ? half : !half) continue;  // the other launch's segment
+    const uint32_t q0 = small_kstart[j], q1 = small_kstart[j + 1];
+    // the ks and their ranks are fetched while the segment loads
+    uint32_t kk = 0, kr = 0;
+    if (threadIdx.x < q1 - q0 && q1 - q0 <= kSelectMaxKs) {
+      kk = klist[q0 + threadIdx.x];
+      kr = k_rank[kk];
+    }
+    if (!full && len > fin_cap / 2) {  // narrow interval: half-width offsets from klo
+      H* sh = reinterpret_cast<H*>(smem);
+      load_segment_lds<K, H>(sh, buf, st, len, klo);
+      finish_segment<T, H>(sh, len, (H)0, (H)(khi - klo), klo, q0, q1, kk, kr, klist, k_rank, k_state, out, L);
     } else {
-      uint32_t M = pow2ceil_u32(len);
-      load_segment_lds<K>(s, buf, st, len);
-      for (uint32_t i = len + threadIdx.x; i < M; i += blockDim.x) s[i] = KeyMax<K>::v;
-      __syncthreads();
-      bitonic_sort_lds<K>(s, M);
-      for (uint32_t q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
-        uint32_t k = klist[q];
-        out[k] = KeyTraits<T>::dec(s[k_rank[k] - 1]);
-        k_state[k] = kDone;
-      }
+      K* sk = reinterpret_cast<K*>(smem);
+      load_segment_lds<K, K>(sk, buf, st, len, (K)0);
+      finish_segment<T, K>(sk, len, klo, khi, (K)0, q0, q1, kk, kr, klist, k_rank, k_state, out, L);
     }
     __syncthreads();
   }

@@ -1,7 +1,8 @@
 // Emulated workgroup runs of the finish kernel's multi-rank radix select
 // (radix_select_multi) against std::sort: segments of 1..16384 keys inside a
 // random key interval, 1..8 ranks each (duplicates allowed), 32- and 64-bit
-// keys, narrow intervals full of duplicates and the full key range.
+// keys and the finish's 16-bit offsets, narrow intervals full of duplicates
+// and the full key range.
 #include "hip/hip_runtime.h"
 #include <algorithm>
 #include <random>
@@ -39,7 +40,7 @@ template <class K> int run(uint64_t seed, uint32_t nt) {
   std::vector<K> sorted = a;
   std::sort(sorted.begin(), sorted.end());
   const uint32_t q = 1 + (uint32_t)(rng() % ggms::kSelectMaxKs);
-  static ggms::SelectLds<K> L;
+  static ggms::SelectLds L;
   std::vector<uint32_t> ranks(q);
   for (auto& r : ranks) r = 1 + (uint32_t)(rng() % len);
   if (q > 1 && (rng() & 1)) ranks[1] = ranks[0];  // a repeated rank
@@ -52,9 +53,9 @@ template <class K> int run(uint64_t seed, uint32_t nt) {
   for (auto& x : th) x.join();
   int bad = 0;
   for (uint32_t j = 0; j < q; ++j)
-    if (L.pre[j] != sorted[ranks[j] - 1]) {
+    if ((K)L.pre[j] != sorted[ranks[j] - 1]) {
       if (bad < 3) printf("FAIL K=%zu len=%u q=%u kind=%d rank=%u got %llx want %llx\n", sizeof(K), len, q, kind, ranks[j],
-                          (unsigned long long)L.pre[j], (unsigned long long)sorted[ranks[j] - 1]);
+                          (unsigned long long)(K)L.pre[j], (unsigned long long)sorted[ranks[j] - 1]);
       ++bad;
     }
   return bad ? 1 : 0;
@@ -65,6 +66,7 @@ int main(int argc, char** argv) {
   for (int i = 0; i < trials; ++i) {
     bad += run<uint32_t>(1000 + i, (i & 1) ? 256 : 128);
     bad += run<uint64_t>(5000 + i, (i & 1) ? 128 : 256);
+    bad += run<uint16_t>(9000 + i, (i & 1) ? 256 : 128);
   }
   printf(bad ? "select FAIL\n" : "select ok\n");
   return bad;

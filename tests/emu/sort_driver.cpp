@@ -34,7 +34,10 @@ int main() {
       bad += run<uint32_t>(M, nt, M + nt);
       bad += run<uint32_t>(M, nt, 3 * (M + nt));
       if (M <= 8192) bad += run<uint64_t>(M, nt, M * 7 + nt);
+      bad += run<uint16_t>(M, nt, M * 5 + nt);
     }
+  bad += run<uint16_t>(32768, 1024, 77);  // 32 keys per thread in registers
+  bad += run<uint16_t>(32768, 1024, 78);
   printf(bad ? "sort FAIL\n" : "sort ok\n");
   return bad;
 }

@@ -233,3 +233,21 @@ def test_many_passes(gpu_ctx, dt, dist):
         st = gpu_ctx.stats()
         assert np.array_equal(_bits(got), _bits(want)), (b, st["passes"])
         assert st["passes"] >= 3
+
+
+@pytest.mark.parametrize("dt", ["uint32", "float64"])
+def test_finish_layouts(gpu_ctx, dt):
+    """The finish holds a segment as keys (at most half its capacity), as
+    half-width offsets from the segment's low key (narrow key interval), or, in a
+    second launch, as keys at full capacity (wide and large).  Sixteen first-pass
+    buckets over uniform values give segments of ~n/16 keys with wide intervals;
+    the tiny path puts everything in one whole-range segment."""
+    import ggms
+    cap = 32768 if dt == "uint32" else 16384
+    for n, dist, b in ((cap // 2, "uniform", 0), (cap // 2 + 1, "uniform", 0), (cap, "uniform", 0),
+                       (int(cap * 9.5), "uniform", 16), (int(cap * 9.5), "normal", 16), (3 * cap, "uniform", 0)):
+        x = ggms.generate(dt, dist, n, 41 + n)
+        ks = ggms.generate_ks("uniform_random", 40, n, 42 + n)
+        want = oracle.sort_and_choose(_host(x), ks, threads=8)
+        got = gpu_ctx.select(x, ks, "iterative", ggms.default_opts(num_buckets=b) if b else None)
+        assert np.array_equal(_bits(got), _bits(want)), (n, dist, b)
