@@ -390,9 +390,8 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
   HIP_TRY(hipGetLastError());
 
   const uint32_t fin_grid = std::max<uint32_t>(1, std::min<uint32_t>(4 * (uint32_t)ctx->cus, 65535));
-  const size_t fin_lds_half = (size_t)(c.fin_cap / 2) * sizeof(K);  // two workgroups per CU
-  const size_t fin_lds_full = (size_t)c.fin_cap * sizeof(K);
-  set_lds_attr((const void*)k_finish<T>, fin_lds_full);
+  const size_t fin_lds = (size_t)(c.fin_cap / 2) * sizeof(K);  // two workgroups per CU
+  set_lds_attr((const void*)k_finish<T>, fin_lds);
 
   if (n <= c.fin_cap) {
     prof.begin(GGMS_KERNEL_OTHER, -1);
@@ -402,12 +401,9 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
     prof.end();
     HIP_TRY(hipGetLastError());
     prof.begin(GGMS_KERNEL_FINISH, 0);
-    // one segment over the whole key range: the half layout holds it as keys
-    // when it has at most fin_cap / 2 of them
-    const uint32_t full = n > c.fin_cap / 2 ? 1u : 0u;
-    k_finish<T><<<1, 1024, full ? fin_lds_full : fin_lds_half, st>>>(
+    k_finish<T><<<1, 1024, fin_lds, st>>>(
         bufA, ctl, (uint32_t*)P(bSmallStart), (uint32_t*)P(bSmallLen), (K*)P(bSmallKlo), (K*)P(bSmallKhi),
-        (uint32_t*)P(bSmallKStart), nullptr, klist, k_rank, k_state, d_out, full, c.fin_cap, dflag);
+        (uint32_t*)P(bSmallKStart), nullptr, klist, k_rank, k_state, d_out, c.fin_cap, dflag);
     prof.end();
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(&hctl[0], ctl, ctlb, hipMemcpyDeviceToHost, st));
@@ -577,15 +573,12 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
       const PassCtl h = hmirror[0];
       const PassCtl hn = hmirror[1];
       if (pass > 0 && h.U_small > 0) {  // the first pass hands every k-bucket to pass 1
-        for (uint32_t full = 0; full < 2; ++full) {
-          if (full && h.n_full == 0) break;
-          prof.begin(GGMS_KERNEL_FINISH, (int)pass);
-          k_finish<T><<<fin_grid, 1024, full ? fin_lds_full : fin_lds_half, st>>>(
-              dst, cp, pl.small_start, pl.small_len, pl.small_klo, pl.small_khi, pl.small_kstart, pl.small_order,
-              klist, k_rank, k_state, d_out, full, c.fin_cap, dflag);
-          prof.end();
-          HIP_TRY(hipGetLastError());
-        }
+        prof.begin(GGMS_KERNEL_FINISH, (int)pass);
+        k_finish<T><<<fin_grid, 1024, fin_lds, st>>>(dst, cp, pl.small_start, pl.small_len, pl.small_klo,
+                                                     pl.small_khi, pl.small_kstart, pl.small_order, klist, k_rank,
+                                                     k_state, d_out, c.fin_cap, dflag);
+        prof.end();
+        HIP_TRY(hipGetLastError());
       }
       if (pass == 0 && (h.bad_k || h.nan_count)) {
         HIP_TRY(hipStreamSynchronize(st));

@@ -59,8 +59,7 @@ struct PassCtl {
   uint32_t bad_k;          // number of ks outside [1, n]
   uint32_t append;         // first-pass compaction: next free position of the survivor buffer
   uint32_t l2_used;        // first pass: level-2 tables of the cell table in use
-  uint32_t n_full;         // small segments that need the finish's full LDS layout
-  uint32_t pad[12];
+  uint32_t pad[13];
 };
 
 struct Plan0Args {
@@ -208,10 +207,10 @@ template <class K> struct HalfKey;
 template <> struct HalfKey<uint32_t> { using T = uint16_t; };
 template <> struct HalfKey<uint64_t> { using T = uint32_t; };
 
-// The finish runs in two layouts.  Half: fin_cap / 2 full-width keys of LDS,
-// which holds a segment of at most fin_cap / 2 keys as keys, or one of at most
-// fin_cap keys whose interval is narrow enough for half-width offsets (two
-// workgroups per CU).  Full: fin_cap keys, for the rest (one per CU).
+// The finish holds a segment in fin_cap / 2 full-width keys of LDS (two
+// workgroups per CU): as keys when it has at most fin_cap / 2 of them, or as
+// half-width offsets from its low key when its interval is that narrow.  The
+// rest (wide and large, rare) are selected straight from global memory.
 template <class K>
 __host__ __device__ __forceinline__ bool finish_fits_half(uint32_t len, K klo, K khi, uint32_t fin_cap) {
   using H = typename HalfKey<K>::T;
@@ -888,7 +887,6 @@ __global__ void __launch_bounds__(1024) k_plan(PlanArgs<typename KeyTraits<T>::K
   uint32_t* P = reinterpret_cast<uint32_t*>(smem);     // inclusive prefix of the histogram
   uint32_t* aidx = P + B;                              // active flag -> index + 1
 
-  if (t == 0) sh_tot[5] = 0;  // small segments for the full finish layout
   __shared__ uint32_t ord_cnt[kOrderBins];  // small segments per size class (finish order)
   if (t < kOrderBins) ord_cnt[t] = 0;
   // P0: histogram prefix, reset tables
@@ -1035,7 +1033,6 @@ __global__ void __launch_bounds__(1024) k_plan(PlanArgs<typename KeyTraits<T>::K
       a.small_klo[j] = klo;
       a.small_khi[j] = khi;
       a.small_kfill[j] = 0;
-      if (!finish_fits_half<K>(c, klo, khi, a.fin_cap)) atomicAdd(&sh_tot[5], 1u);
     }
   }
   __threadfence_block();
@@ -1181,7 +1178,6 @@ __global__ void __launch_bounds__(1024) k_plan(PlanArgs<typename KeyTraits<T>::K
     cc->N_big = N_big;
     cc->N_small = N_small;
     cc->live_ks = live_tot;
-    cc->n_full = sh_tot[5];
     uint32_t nan = 0;
     if (ctl.nan_lo_bucket != kNone32) nan += a.totals[ctl.nan_lo_bucket];
     if (ctl.nan_hi_bucket != kNone32) nan += a.totals[ctl.nan_hi_bucket];
@@ -1542,7 +1538,9 @@ struct SelectLds {
 };
 
 // On entry kid/rank hold q (<= kSelectMaxKs) ks sorted by rank; on exit pre[j]
-// is the key of rank rank[j] among s[0, len).  Whole workgroup.
+// is the key of rank rank[j] among s[0, len).  Whole workgroup.  s is the
+// segment in LDS, or in global memory for a segment too wide and too large
+// for the LDS layouts (each digit then reads it from the caches again).
 template <class K>
 __device__ void radix_select_multi(const K* __restrict__ s, uint32_t len, K klo, K khi, uint32_t q, SelectLds& L) {
   constexpr uint32_t kb = sizeof(K) * 8;
@@ -1570,21 +1568,30 @@ __device__ void radix_select_multi(const K* __restrict__ s, uint32_t len, K klo,
     for (uint32_t i = 0; i < kSelectMaxKs; ++i) gp[i] = (K)L.gpre[i];
     for (uint32_t i = threadIdx.x; i < ng * 256; i += blockDim.x) L.hist[i] = 0;
     __syncthreads();
-    if (ng == 1) {  // every k shares one prefix: the common case
-      for (uint32_t i = threadIdx.x; i < len; i += blockDim.x) {
-        const K key = s[i];
-        lds_count(L.hist, (uint32_t)(key >> shift) & 255u, (K)(key & hmask) == gp[0]);
+    // every key once, four reads in flight per thread (s may be LDS or global)
+    auto sweep = [&](auto&& f) {
+      const uint32_t nt = blockDim.x;
+      uint32_t i = threadIdx.x;
+      for (; i + 3 * nt < len; i += 4 * nt) {
+        const K k0 = s[i], k1 = s[i + nt], k2 = s[i + 2 * nt], k3 = s[i + 3 * nt];
+        f(k0);
+        f(k1);
+        f(k2);
+        f(k3);
       }
+      for (; i < len; i += nt) f(s[i]);
+    };
+    if (ng == 1) {  // every k shares one prefix: the common case
+      sweep([&](K key) { lds_count(L.hist, (uint32_t)(key >> shift) & 255u, (K)(key & hmask) == gp[0]); });
     } else {
-      for (uint32_t i = threadIdx.x; i < len; i += blockDim.x) {
-        const K key = s[i];
+      sweep([&](K key) {
         const K km = (K)(key & hmask);
         uint32_t g = ng;  // the prefixes are distinct: at most one matches
 #pragma unroll
         for (uint32_t p = 0; p < kSelectMaxKs; ++p)
           if (p < ng && gp[p] == km) g = p;
         lds_count(L.hist, g * 256 + ((uint32_t)(key >> shift) & 255u), g < ng);
-      }
+      });
     }
     __syncthreads();
     for (uint32_t j = wv; j < q; j += nw) {
@@ -1645,8 +1652,10 @@ __device__ __forceinline__ void load_segment_lds(C* __restrict__ s, const K* __r
 }
 
 // Answers the ks [q0, q1) of one segment held in LDS as type C, C value = key - base.
+// s is writable LDS, except for the global-memory case, which passes at most
+// kSelectMaxKs ks and so never reaches the sort.
 template <class T, class C>
-__device__ __forceinline__ void finish_segment(C* __restrict__ s, uint32_t len, C clo, C chi,
+__device__ __forceinline__ void finish_segment(const C* __restrict__ s, uint32_t len, C clo, C chi,
                                                typename KeyTraits<T>::K base, uint32_t q0, uint32_t q1,
                                                uint32_t kk, uint32_t kr, const uint32_t* __restrict__ klist,
                                                uint32_t* __restrict__ k_rank, uint32_t* __restrict__ k_state,
@@ -1679,10 +1688,11 @@ __device__ __forceinline__ void finish_segment(C* __restrict__ s, uint32_t len, 
       k_state[k] = kDone;
     }
   } else {
+    C* w = const_cast<C*>(s);
     const uint32_t M = pow2ceil_u32(len);
-    for (uint32_t i = len + threadIdx.x; i < M; i += blockDim.x) s[i] = (C)~(C)0;
+    for (uint32_t i = len + threadIdx.x; i < M; i += blockDim.x) w[i] = (C)~(C)0;
     __syncthreads();
-    bitonic_sort_lds<C>(s, M);
+    bitonic_sort_lds<C>(w, M);
     for (uint32_t qq = q0 + threadIdx.x; qq < q1; qq += blockDim.x) {
       const uint32_t k = klist[qq];
       out[k] = KeyTraits<T>::dec((K)(base + (K)s[k_rank[k] - 1]));
@@ -1691,12 +1701,9 @@ __device__ __forceinline__ void finish_segment(C* __restrict__ s, uint32_t len, 
   }
 }
 
-// Eight waves per SIMD (two workgroups per CU in the half layout) caps the
-// scalar registers: left alone the compiler takes 91 SGPRs, which admits only
-// one 1024-thread workgroup per CU.
-// full == 0: the half layout (dynamic LDS fin_cap / 2 keys) takes every segment
-// finish_fits_half accepts; full != 0: the full layout (fin_cap keys) takes the
-// others.
+// Eight waves per SIMD (two workgroups per CU) caps the scalar registers: left
+// alone the compiler takes 91 SGPRs, which admits only one 1024-thread
+// workgroup per CU.  Dynamic LDS: fin_cap / 2 keys.
 template <class T>
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8))) k_finish(const typename KeyTraits<T>::K* __restrict__ buf, const PassCtl* __restrict__ ctl,
                                                  const uint32_t* __restrict__ small_start,
@@ -1707,7 +1714,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8)
                                                  const uint32_t* __restrict__ small_order,
                                                  const uint32_t* __restrict__ klist, uint32_t* __restrict__ k_rank,
                                                  uint32_t* __restrict__ k_state, T* __restrict__ out,
-                                                 uint32_t full, uint32_t fin_cap, uint32_t* __restrict__ err) {
+                                                 uint32_t fin_cap, uint32_t* __restrict__ err) {
   using K = typename KeyTraits<T>::K;
   using H = typename HalfKey<K>::T;
   extern __shared__ __align__(16) unsigned char smem[];
@@ -1721,16 +1728,28 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8)
       continue;
     }
     const K klo = small_klo[j], khi = small_khi[j];
-    const bool half = finish_fits_half<K>(len, klo, khi, fin_cap);
-    if (full ? half : !half) continue;  // the other launch's segment
     const uint32_t q0 = small_kstart[j], q1 = small_kstart[j + 1];
+    if (!finish_fits_half<K>(len, klo, khi, fin_cap)) {
+      // wide and large: radix select from global memory, eight ks at a time
+      for (uint32_t c0 = q0; c0 < q1; c0 += kSelectMaxKs) {
+        const uint32_t c1 = q1 - c0 < kSelectMaxKs ? q1 : c0 + kSelectMaxKs;
+        uint32_t kk = 0, kr = 0;
+        if (threadIdx.x < c1 - c0) {
+          kk = klist[c0 + threadIdx.x];
+          kr = k_rank[kk];
+        }
+        finish_segment<T, K>(buf + st, len, klo, khi, (K)0, c0, c1, kk, kr, klist, k_rank, k_state, out, L);
+        __syncthreads();
+      }
+      continue;
+    }
     // the ks and their ranks are fetched while the segment loads
     uint32_t kk = 0, kr = 0;
     if (threadIdx.x < q1 - q0 && q1 - q0 <= kSelectMaxKs) {
       kk = klist[q0 + threadIdx.x];
       kr = k_rank[kk];
     }
-    if (!full && len > fin_cap / 2) {  // narrow interval: half-width offsets from klo
+    if (len > fin_cap / 2) {  // narrow interval: half-width offsets from klo
       H* sh = reinterpret_cast<H*>(smem);
       load_segment_lds<K, H>(sh, buf, st, len, klo);
       finish_segment<T, H>(sh, len, (H)0, (H)(khi - klo), klo, q0, q1, kk, kr, klist, k_rank, k_state, out, L);
