@@ -511,7 +511,6 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
       pl.small_klo = (K*)P(bSmallKlo);
       pl.small_khi = (K*)P(bSmallKhi);
       pl.small_kstart = (uint32_t*)P(bSmallKStart);
-      pl.small_kfill = (uint32_t*)P(bSmallKFill);
       pl.small_order = (uint32_t*)P(bSmallOrder);
       pl.klist = klist;
       pl.wg_first_next = wgf_next;
@@ -526,7 +525,7 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
       pl.next_params = pass == 0 ? (MapParam<K>*)P(bParams1) : nullptr;
       pl.rank_words = pass == 0 ? (uint64_t*)P(bRank0) : nullptr;
       pl.mirror = dmirror;
-      size_t lds_plan = (size_t)B_bound * 8;
+      size_t lds_plan = (size_t)B_bound * 8 + (size_t)rank_word_count(B_bound) * 8;
       LDS_CHECK(lds_plan);
       set_lds_attr((const void*)k_plan<T>, lds_plan);
       prof.begin(GGMS_KERNEL_PLAN, (int)pass);

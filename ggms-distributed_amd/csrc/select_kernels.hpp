@@ -158,26 +158,33 @@ __device__ __forceinline__ void bitonic_reg_merge(K (&r)[E], uint32_t base, uint
   }
 }
 
+// Threads past M / E hold no keys (they only take part in the barriers and
+// the LDS stages).
 template <class K, int E>
 __device__ void bitonic_sort_lds_reg(K* s, uint32_t M) {
   const uint32_t base = threadIdx.x * E;
+  const bool own = base < M;
   K r[E];
+  if (own) {
 #pragma unroll
-  for (int i = 0; i < E; ++i) r[i] = s[base + i];
-  for (uint32_t size = 2; size <= (uint32_t)E; size <<= 1) bitonic_reg_merge<K, E>(r, base, size, size >> 1);
+    for (int i = 0; i < E; ++i) r[i] = s[base + i];
+    for (uint32_t size = 2; size <= (uint32_t)E; size <<= 1) bitonic_reg_merge<K, E>(r, base, size, size >> 1);
 #pragma unroll
-  for (int i = 0; i < E; ++i) s[base + i] = r[i];
+    for (int i = 0; i < E; ++i) s[base + i] = r[i];
+  }
   __syncthreads();
   for (uint32_t size = 2 * E; size <= M; size <<= 1) {
     for (uint32_t stride = size >> 1; stride >= (uint32_t)E; stride >>= 1) {
       bitonic_stage_lds<K>(s, M, size, stride);
       __syncthreads();
     }
+    if (own) {
 #pragma unroll
-    for (int i = 0; i < E; ++i) r[i] = s[base + i];
-    bitonic_reg_merge<K, E>(r, base, size, E / 2);
+      for (int i = 0; i < E; ++i) r[i] = s[base + i];
+      bitonic_reg_merge<K, E>(r, base, size, E / 2);
 #pragma unroll
-    for (int i = 0; i < E; ++i) s[base + i] = r[i];
+      for (int i = 0; i < E; ++i) s[base + i] = r[i];
+    }
     __syncthreads();
   }
 }
@@ -191,6 +198,10 @@ __device__ void bitonic_sort_lds(K* s, uint32_t M) {
     else if (e == 8) bitonic_sort_lds_reg<K, 8>(s, M);
     else if (e == 16) bitonic_sort_lds_reg<K, 16>(s, M);
     else bitonic_sort_lds_reg<K, 32>(s, M);
+    return;
+  }
+  if (e < 4 && M >= 256 && (M & (M - 1)) == 0) {  // fewer keys than 4 per thread: M / 4 threads hold 4 each
+    bitonic_sort_lds_reg<K, 4>(s, M);
     return;
   }
   for (uint32_t size = 2; size <= M; size <<= 1) {
@@ -686,6 +697,86 @@ __device__ uint32_t build_lut(const K* starts, uint32_t R, K* __restrict__ g_lut
   return used;
 }
 
+// Same table as build_lut, with every level-1 cell decided in O(1) from
+// counts instead of two binary searches: each range start is counted in its
+// cell (16-bit counters, two per word) or, when it sits exactly on the cell's
+// low edge, flagged in a bitmap; an exclusive scan of the per-cell totals then
+// gives, for cell c with P starts in earlier cells,
+//   r0 = P + edge(c) - 1 (last start <= the cell's low key),
+//   r1 = r0 + inside(c)  (last start <= the cell's high key).
+// cw: LDS scratch of lut_count_words<K>() words.  Returns the level-2 tables
+// in use (same value in every thread).
+template <class K> __host__ __device__ constexpr uint32_t lut_count_words() {
+  return LutGeom<K>::L1CELLS / 2 + LutGeom<K>::L1CELLS / 32;
+}
+template <class K>
+__device__ uint32_t build_lut_counted(const K* starts, uint32_t R, K* __restrict__ g_lut, uint32_t* cw) {
+  using G = LutGeom<K>;
+  constexpr uint32_t C = G::L1CELLS;
+  uint32_t* edge = cw + C / 2;
+  __shared__ uint32_t tab_cell[G::L2TABLES];
+  __shared__ uint32_t scan_s[32];
+  const uint32_t nt = blockDim.x, t = threadIdx.x;
+  for (uint32_t i = t; i < lut_count_words<K>(); i += nt) cw[i] = 0;
+  __syncthreads();
+  for (uint32_t r = t; r < R; r += nt) {
+    const K st = starts[r];
+    const uint32_t c = (uint32_t)(st >> G::S1);
+    if (st & G::M1) atomicAdd(&cw[c >> 1], 1u << (16 * (c & 1)));
+    else atomicOr(&edge[c >> 5], 1u << (c & 31));
+  }
+  __syncthreads();
+  auto inside = [&](uint32_t c) { return (cw[c >> 1] >> (16 * (c & 1))) & 0xFFFFu; };
+  auto on_edge = [&](uint32_t c) { return (edge[c >> 5] >> (c & 31)) & 1u; };
+  const uint32_t per = (C + nt - 1) / nt;
+  const uint32_t c0 = t * per, c1 = c0 + per < C ? c0 + per : C;
+  uint32_t nst = 0, nmulti = 0;
+  for (uint32_t c = c0; c < c1; ++c) {
+    const uint32_t k = inside(c);
+    nst += k + on_edge(c);
+    nmulti += k >= 2;
+  }
+  uint32_t tot;
+  const uint32_t ex = block_excl_scan_u32((nmulti << 16) | nst, scan_s, &tot);  // both fit 16 bits
+  uint32_t P = ex & 0xFFFFu, tix = ex >> 16;
+  for (uint32_t c = c0; c < c1; ++c) {
+    const uint32_t k = inside(c), a = on_edge(c);
+    const uint32_t r0 = P + a - 1, r1 = r0 + k;
+    const K lo = (K)((K)c << G::S1);
+    K e;
+    if (k == 0) {
+      e = ((K)r0 << G::S1) | G::M1;
+    } else if (k == 1) {
+      e = ((K)r0 << G::S1) | (K)(starts[r1] - lo - 1);
+    } else if (tix < G::L2TABLES) {
+      tab_cell[tix] = c;
+      e = G::FLAG | G::TABLE | (K)tix;
+      tix++;
+    } else {
+      e = G::FLAG | (K)r0 | ((K)r1 << 11);
+    }
+    g_lut[c] = e;
+    P += a + k;
+  }
+  __syncthreads();
+  // level 2: 64 sub-cells of each tabled cell (few: binary search)
+  const uint32_t nm = tot >> 16;
+  const uint32_t used = nm < G::L2TABLES ? nm : G::L2TABLES;
+  for (uint32_t i = t; i < used * 64; i += nt) {
+    uint32_t tb = i >> 6, sub = i & 63;
+    uint32_t c = tab_cell[tb];
+    K lo = (K)(((K)c << G::S1) | ((K)sub << G::S2)), hi = (K)(lo + G::M2);
+    uint32_t r0 = last_start_le<K>(starts, 0, R - 1, lo);
+    uint32_t r1 = last_start_le<K>(starts, r0, R - 1, hi);
+    K e;
+    if (r1 == r0) e = ((K)r0 << G::S2) | G::M2;
+    else if (r1 == r0 + 1) e = ((K)r0 << G::S2) | (K)(starts[r1] - lo - 1);
+    else e = G::FLAG | (K)r0 | ((K)r1 << 11);
+    g_lut[G::L1CELLS + i] = e;
+  }
+  return used;
+}
+
 // Plain fill (used instead of hipMemsetAsync so the sequence stays capturable).
 static __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t words, uint32_t v) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -858,7 +949,6 @@ template <class K> struct PlanArgs {
   K* small_klo;              // key interval of each small segment
   K* small_khi;
   uint32_t* small_kstart;    // [U_small + 1]
-  uint32_t* small_kfill;     // [U_small]
   uint32_t* small_order;     // [U_small] segments in finish order
   uint32_t* klist;           // [nk]
   uint32_t* wg_first_next;   // [G_max]
@@ -886,10 +976,18 @@ __global__ void __launch_bounds__(1024) k_plan(PlanArgs<typename KeyTraits<T>::K
   const uint32_t B = ctl.B;
   uint32_t* P = reinterpret_cast<uint32_t*>(smem);     // inclusive prefix of the histogram
   uint32_t* aidx = P + B;                              // active flag -> index + 1
+  uint32_t* rk_bits = aidx + B;                        // rank words under construction (first pass)
+  uint32_t* rk_first = rk_bits + rank_word_count(B);
+  uint32_t* kfill = P;  // from P5 on (P and its reuse as range bases are dead): ks per small segment
 
   __shared__ uint32_t ord_cnt[kOrderBins];  // small segments per size class (finish order)
   if (t < kOrderBins) ord_cnt[t] = 0;
   // P0: histogram prefix, reset tables
+  if (a.rank_words)
+    for (uint32_t w = t; w < rank_word_count(B); w += nt) {
+      rk_bits[w] = 0;
+      rk_first[w] = kNone32;
+    }
   for (uint32_t i = t; i < B; i += nt) {
     P[i] = a.totals[i];
     aidx[i] = 0;
@@ -1012,6 +1110,10 @@ __global__ void __launch_bounds__(1024) k_plan(PlanArgs<typename KeyTraits<T>::K
     a.slot_of_bucket[b] = (uint16_t)s;
     a.slot_dst[s] = dst;
     a.slot_bucket[s] = b;
+    if (a.rank_words) {
+      atomicOr(&rk_bits[b >> 5], 1u << (b & 31));
+      atomicMin(&rk_first[b >> 5], s);
+    }
     if (s < U_big) {
       Range<K> nr;
       nr.lo = a.act_klo[q];
@@ -1032,26 +1134,18 @@ __global__ void __launch_bounds__(1024) k_plan(PlanArgs<typename KeyTraits<T>::K
       a.small_len[j] = c;
       a.small_klo[j] = klo;
       a.small_khi[j] = khi;
-      a.small_kfill[j] = 0;
+      kfill[j] = 0;
     }
   }
   __threadfence_block();
   __syncthreads();
 
   // slot rank words (32 buckets each): has-a-slot bits, slot of the first
-  if (a.rank_words) {
+  if (a.rank_words)
     for (uint32_t w = t; w < rank_word_count(B); w += nt) {
-      uint32_t bits = 0, first = 0;
-      for (uint32_t i = 0; i < 32 && w * 32 + i < B; ++i) {
-        uint32_t sv = a.slot_of_bucket[w * 32 + i];
-        if (sv != kNone16) {
-          if (!bits) first = sv;
-          bits |= 1u << i;
-        }
-      }
-      a.rank_words[w] = ((uint64_t)first << 32) | bits;
+      const uint32_t bits = rk_bits[w];
+      a.rank_words[w] = ((uint64_t)(bits ? rk_first[w] : 0u) << 32) | bits;
     }
-  }
 
   // P6: bucket bases of the next pass
   uint32_t B_next = 0;
@@ -1092,7 +1186,7 @@ __global__ void __launch_bounds__(1024) k_plan(PlanArgs<typename KeyTraits<T>::K
       a.k_rank[k] = local;
       a.k_aux[k] = j;
       a.k_state[k] = kSmallNew;
-      atomicAdd(&a.small_kfill[j], 1u);
+      atomicAdd(&kfill[j], 1u);
     }
   }
   __threadfence_block();
@@ -1104,15 +1198,15 @@ __global__ void __launch_bounds__(1024) k_plan(PlanArgs<typename KeyTraits<T>::K
     uint32_t j0 = t * per, j1 = j0 + per < U_small ? j0 + per : U_small;
     uint32_t s = 0;
     for (uint32_t j = j0; j < j1; ++j) {
-      s += a.small_kfill[j];
+      s += kfill[j];
       atomicAdd(&ord_cnt[order_bin(a.small_len[j])], 1u);
     }
     uint32_t tot;
     uint32_t off = block_excl_scan_u32(s, scratch, &tot);
     for (uint32_t j = j0; j < j1; ++j) {
       a.small_kstart[j] = off;
-      off += a.small_kfill[j];
-      a.small_kfill[j] = 0;
+      off += kfill[j];
+      kfill[j] = 0;
     }
     if (t == 0) a.small_kstart[U_small] = tot;
   }
@@ -1131,7 +1225,7 @@ __global__ void __launch_bounds__(1024) k_plan(PlanArgs<typename KeyTraits<T>::K
     uint32_t st = a.k_state[k];
     if (st == kSmallNew) {
       uint32_t j = a.k_aux[k];
-      uint32_t pos = atomicAdd(&a.small_kfill[j], 1u);
+      uint32_t pos = atomicAdd(&kfill[j], 1u);
       a.klist[a.small_kstart[j] + pos] = k;
       a.k_state[k] = kSmall;
     }

@@ -36,11 +36,15 @@ constexpr uint32_t kPlan0MaxRanges = 2064;
 // learned pivots are evaluated in chunks of this many (features of a chunk in LDS)
 constexpr uint32_t kMlpChunk = 128;
 constexpr uint32_t kMlpFeatStride = kMlpIn + 1;
-template <class K> __host__ __device__ constexpr size_t plan0_lds_bytes(uint32_t S_pad) {
+template <class K> __host__ __device__ constexpr size_t plan0_lds_core(uint32_t S_pad) {
   return (size_t)S_pad * sizeof(K) + 2048 * sizeof(K) + kPlan0MaxRanges * sizeof(K) +
          (kPlan0MaxRanges * 8 > (kMlpParams + kMlpChunk * kMlpFeatStride) * 4
               ? kPlan0MaxRanges * 8
               : (kMlpParams + kMlpChunk * kMlpFeatStride) * 4);
+}
+// plus the cell-table counters
+template <class K> __host__ __device__ constexpr size_t plan0_lds_bytes(uint32_t S_pad) {
+  return plan0_lds_core<K>(S_pad) + (size_t)lut_count_words<K>() * 4;
 }
 
 template <class K>
@@ -143,6 +147,7 @@ __global__ void __launch_bounds__(1024) k_plan0(const T* __restrict__ x, Plan0Ar
   const uint32_t Spad = pow2ceil_u32(S);
   K* s = reinterpret_cast<K*>(smem);
   K* q = s + Spad;  // pivot keys, up to 2048, then the range list
+  uint32_t* cells = reinterpret_cast<uint32_t*>(smem + plan0_lds_core<K>(Spad));  // cell-table counters
   const K lowest = KT::kFloat ? KT::kNegInf : (K)0;
   const K highest = KT::kFloat ? KT::kPosInf : KeyMax<K>::v;
 
@@ -338,7 +343,7 @@ __global__ void __launch_bounds__(1024) k_plan0(const T* __restrict__ x, Plan0Ar
     ctl->nan_hi_bucket = KT::kFloat ? Btot - 1 : kNone32;
   }
   // 5. key-prefix cell table, from the LDS copy of the range starts
-  const uint32_t l2 = build_lut<K>(rlo, R, lut);
+  const uint32_t l2 = build_lut_counted<K>(rlo, R, lut, cells);
   if (threadIdx.x == 0) ctl->l2_used = l2;
 }
 

@@ -71,6 +71,13 @@ template <class T> inline T atomicAdd(T* p, T v) { return std::atomic_ref<T>(*p)
 inline float atomicAdd(float* p, float v) { return std::atomic_ref<float>(*p).fetch_add(v); }
 template <class T> inline T atomicOr(T* p, T v) { return std::atomic_ref<T>(*p).fetch_or(v); }
 template <class T> inline T atomicExch(T* p, T v) { return std::atomic_ref<T>(*p).exchange(v); }
+template <class T> inline T atomicMin(T* p, T v) {
+  std::atomic_ref<T> a(*p);
+  T cur = a.load();
+  while (v < cur && !a.compare_exchange_weak(cur, v)) {
+  }
+  return cur;
+}
 inline uint32_t __umulhi(uint32_t a, uint32_t b) { return (uint32_t)(((uint64_t)a * b) >> 32); }
 inline uint64_t __ballot(int p) { return p ? 1ull : 0ull; }
 inline uint32_t __lane_id() { return threadIdx.x & 63; }

@@ -78,6 +78,7 @@ int run(uint32_t mode, uint32_t S, uint32_t P, uint32_t B, int kind, uint64_t se
 }
 int main() {
   int bad = 0;
+
   for (int kind = 0; kind < 3; ++kind) {
     bad += run(kModeLinear, 1024, 17, 8192, kind, 1 + kind);
     bad += run(kModeSketch, 8192, 256, 16384, kind, 11 + kind);
