@@ -1376,11 +1376,12 @@ __global__ void __launch_bounds__(1024) k_compact0(const T* __restrict__ x, cons
           uint32_t first = (uint32_t)__builtin_ctzll(wave_ballot(true));
           uint32_t base = 0;
           if (lane_id() == first) base = atomicAdd(&cursor, tot);
-          base = readlane_u32(base, first);
+          // wave-uniform run pointer: each store is scalar base + 32-bit lane offset
+          K* run = out + readlane_u32(base, first);
 #pragma unroll
           for (int j = 0; j < N; ++j) {
-            if (act[j]) out[base + mask_below(m[j])] = k[j];
-            base += (uint32_t)__builtin_popcountll(m[j]);
+            if (act[j]) run[mask_below(m[j])] = k[j];
+            run += (uint32_t)__builtin_popcountll(m[j]);
           }
         }
       },
