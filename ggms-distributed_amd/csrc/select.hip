@@ -79,12 +79,13 @@ static Config resolve_config(ggms_alg alg, uint64_t n, uint32_t nk, const ggms_o
   if (alg == GGMS_ALG_DISTRIBUTED_1GPU) c.mode = kModeStrided;
   if (o.splitter == GGMS_SPLIT_LEARNED) c.mode = kModeLearned;
   if (o.splitter == GGMS_SPLIT_SKETCH) c.mode = kModeSketch;
-  // 0 = automatic: 8192 (the reference's count) for every splitter.  With
-  // 8192 buckets the first pass's histogram and cell table leave room for two
-  // streaming workgroups per CU; 16384 buckets halve the survivors of the
-  // quantile splitters but the single workgroup per CU costs more than that
-  // saves (measured 1.36 against 1.32 ms at 2^28 fp32, 1001 ks).
-  c.B = o.num_buckets ? o.num_buckets : 8192u;
+  // 0 = automatic: 14336 for every splitter (the reference uses 8192).  More
+  // first-pass buckets leave fewer survivors for pass 1 and the finish; the
+  // first-pass count keeps its speed at one workgroup per CU, and above ~14K
+  // the planning kernel and the column sums grow faster than pass 1 shrinks.
+  // Measured at 2^28 fp32, 1001 ks, linear splitter: 8192 -> 0.802 ms,
+  // 12288 -> 0.757, 14336 -> 0.750, 16384 -> 0.780.
+  c.B = o.num_buckets ? o.num_buckets : 14336u;
   c.B = std::max<uint32_t>(16, std::min<uint32_t>(c.B, kMaxBuckets));
   c.B_cap = kMaxBuckets;
   if (c.mode == kModeLinear || c.mode == kModeStrided) {
@@ -220,9 +221,9 @@ struct Prof {
 };
 
 enum Buf {
-  bBufA, bBufB, bRows, bTotals, bRanges0, bRanges1, bRangesFirst, bCtl, bKs, bKRank, bKState, bKAux, bKList,
+  bBufA, bBufB, bRows, bTotals, bRanges0, bRanges1, bRangesFirst, bCtl, bKRank, bKState, bKAux, bKList,
   bSlotOfBucket, bSlotDst, bSlotBucket, bActBucket, bActCls, bActDst, bActSlot, bActKlo, bActKhi,
-  bSmallStart, bSmallLen, bSmallKStart, bSmallKFill, bWgFirst, bOut, bLut, bSmallKlo, bSmallKhi, bParams1, bActCnt, bRank0, bSmallOrder, bCount
+  bSmallStart, bSmallLen, bSmallKStart, bWgFirst, bLut, bSmallKlo, bSmallKhi, bParams1, bActCnt, bRank0, bSmallOrder, bCount
 };
 static_assert(bCount <= kMaxBuffers, "workspace layout table too small");
 
@@ -242,7 +243,6 @@ static Layout make_layout(uint64_t n, uint32_t nk, const Config& c) {
   sz[bRanges1] = Rm * sizeof(Range<K>);
   sz[bRangesFirst] = ((size_t)c.R0_bound + 64) * sizeof(Range<K>);
   sz[bCtl] = (size_t)(c.max_passes + 2) * sizeof(PassCtl);
-  sz[bKs] = nkp * 4;
   sz[bKRank] = nkp * 4;
   sz[bKState] = nkp * 4;
   sz[bKAux] = nkp * 4;
@@ -261,10 +261,8 @@ static Layout make_layout(uint64_t n, uint32_t nk, const Config& c) {
   sz[bSmallStart] = Bm * 4;
   sz[bSmallLen] = Bm * 4;
   sz[bSmallKStart] = (Bm + 1) * 4;
-  sz[bSmallKFill] = Bm * 4;
   sz[bSmallOrder] = Bm * 4;
   sz[bWgFirst] = (size_t)c.G_max * 4 * 2;  // double-buffered by pass parity
-  sz[bOut] = nkp * 8;
   sz[bLut] = LutGeom<K>::bytes();
   sz[bParams1] = Bm * sizeof(MapParam<K>);
   sz[bSmallKlo] = Bm * sizeof(K);
@@ -326,7 +324,8 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
   ggms_status s = ensure_ws(ctx, L.total);
   if (s) return s;
   const size_t ctlb = sizeof(PassCtl);
-  s = ensure_pinned(ctx, 6 * ctlb + 256 + (size_t)nk * 8 + 4096);
+  const size_t ks_bytes = ((size_t)nk * 4 + 255) / 256 * 256;
+  s = ensure_pinned(ctx, 6 * ctlb + 256 + ks_bytes + (size_t)nk * sizeof(T) + 4096);
   if (s) return s;
   char* W = (char*)ctx->ws.base;
   auto P = [&](int b) { return (void*)(W + L.off[b]); };
@@ -337,16 +336,16 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
   Range<K>* rng[2] = {(Range<K>*)P(bRanges0), (Range<K>*)P(bRanges1)};  // passes >= 1, by parity
   Range<K>* rng_first = (Range<K>*)P(bRangesFirst);                      // pass 0, kept for pass 1
   PassCtl* ctl = (PassCtl*)P(bCtl);
-  uint32_t* d_ks = (uint32_t*)P(bKs);
+
   uint32_t* k_rank = (uint32_t*)P(bKRank);
   uint32_t* k_state = (uint32_t*)P(bKState);
   uint32_t* k_aux = (uint32_t*)P(bKAux);
   uint32_t* klist = (uint32_t*)P(bKList);
-  T* d_out = out_on_device ? (T*)out : (T*)P(bOut);
 
-  // pinned host memory: ctl blocks [0, 4) for copies, the plan's mirror of the
-  // current and next control blocks [4, 6), the finish's error flag, then ks in
-  // and outputs out
+  // pinned host memory (the kernels read and write it directly): ctl blocks
+  // [0, 4) for copies, the plan's mirror of the current and next control
+  // blocks [4, 6), the finish's error flag and the bad-k flag, then the ks
+  // (read by k_setup) and the outputs (written by the plan and the finish)
   char* H = (char*)ctx->pinned;
   char* Hd = (char*)ctx->pinned_dev;
   PassCtl* hctl = (PassCtl*)H;
@@ -354,8 +353,15 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
   PassCtl* dmirror = (PassCtl*)(Hd + 4 * ctlb);
   volatile uint32_t* hflag = (volatile uint32_t*)(H + 6 * ctlb);
   uint32_t* dflag = (uint32_t*)(Hd + 6 * ctlb);
-  char* hdata = H + 6 * ctlb + 256;
+  volatile uint32_t* hbad = hflag + 1;
+  uint32_t* dbad = dflag + 1;
+  char* hks = H + 6 * ctlb + 256;
+  const uint32_t* d_ks = ks_on_device ? ks : (const uint32_t*)(Hd + 6 * ctlb + 256);
+  char* hout = hks + ks_bytes;
+  T* d_out = out_on_device ? (T*)out : (T*)(Hd + 6 * ctlb + 256 + ks_bytes);
   *hflag = 0;
+  *hbad = 0;
+  if (!ks_on_device) memcpy(hks, ks, (size_t)nk * 4);
 
   // control block 0
   PassCtl c0{};
@@ -368,24 +374,15 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
   c0.G = G0;
   c0.nan_lo_bucket = kNone32;
   c0.nan_hi_bucket = kNone32;
-  memcpy(&hctl[2], &c0, ctlb);
-  {
-    uint32_t words = (uint32_t)((c.max_passes + 2) * ctlb / 4);
-    k_fill_u32<<<(words + 255) / 256, 256, 0, st>>>((uint32_t*)ctl, words, 0u);
-    HIP_TRY(hipGetLastError());
-  }
-  HIP_TRY(hipMemcpyAsync(ctl, &hctl[2], ctlb, hipMemcpyHostToDevice, st));
-  if (ks_on_device) {
-    HIP_TRY(hipMemcpyAsync(d_ks, ks, (size_t)nk * 4, hipMemcpyDeviceToDevice, st));
-  } else {
-    memcpy(hdata, ks, (size_t)nk * 4);
-    HIP_TRY(hipMemcpyAsync(d_ks, hdata, (size_t)nk * 4, hipMemcpyHostToDevice, st));
-  }
   ggms_stats& stt = ctx->stats;
   memset(&stt, 0, sizeof(stt));
   Prof prof(ctx, st, o.profile != 0);
   prof.begin(GGMS_KERNEL_OTHER, -1);
-  k_init_ks<<<(nk + 255) / 256, 256, 0, st>>>(d_ks, nk, n, k_rank, k_state, ctl);
+  {
+    const uint32_t words = (uint32_t)((c.max_passes + 2) * ctlb / 4);
+    const uint32_t threads = std::max(words, nk);
+    k_setup<<<(threads + 255) / 256, 256, 0, st>>>(d_ks, nk, n, k_rank, k_state, ctl, words, c0, dbad);
+  }
   prof.end();
   HIP_TRY(hipGetLastError());
 
@@ -406,9 +403,8 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
         (uint32_t*)P(bSmallKStart), nullptr, klist, k_rank, k_state, d_out, c.fin_cap, dflag);
     prof.end();
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipMemcpyAsync(&hctl[0], ctl, ctlb, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
-    if (hctl[0].bad_k) return GGMS_EINVAL;
+    if (*hbad) return GGMS_EINVAL;
     if (*hflag) return GGMS_EDEVICE;
     stt.passes = 0;
     stt.finished_segments = 1;
@@ -579,9 +575,9 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
         prof.end();
         HIP_TRY(hipGetLastError());
       }
-      if (pass == 0 && (h.bad_k || h.nan_count)) {
+      if (pass == 0 && (*hbad || h.nan_count)) {
         HIP_TRY(hipStreamSynchronize(st));
-        return h.bad_k ? GGMS_EINVAL : GGMS_ENAN;
+        return *hbad ? GGMS_EINVAL : GGMS_ENAN;
       }
       if (pass == 0) {
         B0 = h.B;
@@ -607,13 +603,8 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
       U_slot_bound = std::min<uint32_t>(nk, B_bound);
     }
   }
-  if (!out_on_device) {
-    HIP_TRY(hipMemcpyAsync(hdata, d_out, (size_t)nk * sizeof(T), hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipStreamSynchronize(st));
-    memcpy(out, hdata, (size_t)nk * sizeof(T));
-  } else {
-    HIP_TRY(hipStreamSynchronize(st));
-  }
+  HIP_TRY(hipStreamSynchronize(st));
+  if (!out_on_device) memcpy(out, hout, (size_t)nk * sizeof(T));
   if (*hflag) return GGMS_EDEVICE;  // a segment over the finish capacity (never expected)
   prof.collect(stt);
   return GGMS_OK;

@@ -56,10 +56,9 @@ struct PassCtl {
   uint32_t live_ks;
   uint32_t nan_lo_bucket;  // bucket holding keys below key(-inf), or kNone32
   uint32_t nan_hi_bucket;  // bucket holding keys above key(+inf), or kNone32
-  uint32_t bad_k;          // number of ks outside [1, n]
   uint32_t append;         // first-pass compaction: next free position of the survivor buffer
   uint32_t l2_used;        // first pass: level-2 tables of the cell table in use
-  uint32_t pad[13];
+  uint32_t pad[14];
 };
 
 struct Plan0Args {
@@ -784,15 +783,22 @@ static __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t words, uint
 }
 
 // ---------------------------------------------------------------------------
-// k_init_ks: k-th largest (1-based) -> k-th smallest rank; validates range.
+// k_setup: the first kernel of a call.  Zeroes the control blocks and writes
+// block 0 from the host's values; turns every k (k-th largest, 1-based) into a
+// k-th smallest rank.  A k outside [1, n] sets *bad (a flag word in
+// host-visible memory) and is never selected.  ks may be host-visible memory.
 // ---------------------------------------------------------------------------
-static __global__ void k_init_ks(const uint32_t* __restrict__ ks, uint32_t nk, uint32_t n, uint32_t* __restrict__ k_rank,
-                          uint32_t* __restrict__ k_state, PassCtl* ctl0) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+static __global__ void k_setup(const uint32_t* __restrict__ ks, uint32_t nk, uint32_t n, uint32_t* __restrict__ k_rank,
+                               uint32_t* __restrict__ k_state, PassCtl* __restrict__ ctl, uint32_t ctl_words,
+                               PassCtl c0, uint32_t* __restrict__ bad) {
+  constexpr uint32_t kCtlWords = sizeof(PassCtl) / 4;
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) ctl[0] = c0;
+  else if (i >= kCtlWords && i < ctl_words) reinterpret_cast<uint32_t*>(ctl)[i] = 0u;
   if (i >= nk) return;
   uint32_t k = ks[i];
   if (k < 1 || k > n) {
-    atomicAdd(&ctl0->bad_k, 1u);
+    *bad = 1u;
     k_rank[i] = 1;
     k_state[i] = kDone;
     return;
@@ -1122,8 +1128,10 @@ __global__ void __launch_bounds__(1024) k_plan(PlanArgs<typename KeyTraits<T>::K
       nr.seg_end = dst + c;
       nr.base = 0;
       uint32_t nb;
-      if (c <= a.fin_cap) nb = 1;  // merged small bucket: carried whole, finished after the next pass
-      else if (2 * U_big <= Bn) nb = 2 + (uint32_t)(((uint64_t)c * (Bn - 2 * U_big)) / N_big);
+      // small buckets carried into the next pass (merge_small) are split like
+      // the others: their elements pass through the next pass anyway, and
+      // finer buckets leave less for the finish
+      if (2 * U_big <= Bn) nb = 2 + (uint32_t)(((uint64_t)c * (Bn - 2 * U_big)) / N_big);
       else nb = 1 + (uint32_t)(((uint64_t)c * (Bn - U_big)) / N_big);
       make_map<K>(nr, nb);
       a.next_ranges[s] = nr;

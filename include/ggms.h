@@ -52,7 +52,7 @@ typedef enum {
 
 /* Tunables.  Zero means "library default" for every numeric field. */
 typedef struct {
-  uint32_t num_buckets;   /* buckets per pass (0 = auto: 8192; max 16384) */
+  uint32_t num_buckets;   /* buckets per pass (0 = auto: 14336; max 16384; the reference uses 8192) */
   uint32_t num_pivots;    /* linear splitter pivots (default 17 => 16 ranges) */
   uint32_t sample_size;   /* splitter sample (default 1024; 8192 for the one-GPU distributed form) */
   uint32_t block_threads; /* workgroup size of the streaming kernels (default 1024) */
