@@ -71,9 +71,12 @@ def bench_other_configs(ctx, args):
             ctx.select(x, ks, "iterative", o)
         torch.cuda.synchronize()
         ms = (time.perf_counter() - t0) / reps * 1e3
+        ctx.select(x, ks, "iterative", ggms.default_opts(splitter="linear", profile=1))
         st = ctx.stats()
         out.append({"workload": f"{label}, N=2^{lg}, {nk} ks {kd}", "ms_per_step": round(ms, 4),
-                    "elements_per_s": round(n / (ms * 1e-3), 1), "passes": st["passes"]})
+                    "elements_per_s": round(n / (ms * 1e-3), 1), "passes": st["passes"],
+                    "n_kept": st["n_kept"], "buckets": st["buckets"],
+                    "kernel_ms": {k: round(v, 4) for k, v in st["kernel_ms"].items() if v > 0}})
         del x
     return out
 

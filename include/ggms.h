@@ -56,7 +56,7 @@ typedef struct {
   uint32_t num_pivots;    /* linear splitter pivots (default 17 => 16 ranges) */
   uint32_t sample_size;   /* splitter sample (default 1024; 8192 for the one-GPU distributed form) */
   uint32_t block_threads; /* workgroup size of the streaming kernels (default 1024) */
-  uint32_t blocks_per_cu; /* streaming workgroups per CU (default 2) */
+  uint32_t blocks_per_cu; /* streaming workgroups per CU (0 = auto: 2 in the first pass, 1 after) */
   uint32_t splitter;      /* ggms_splitter (default LINEAR) */
   uint32_t learned_pieces;/* ranges produced by the learned / sketch splitter (default 64) */
   uint32_t max_passes;    /* safety cap on bucketing passes (default 64) */
