@@ -395,16 +395,21 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
   set_lds_attr((const void*)k_finish<T>, fin_lds);
 
   if (n <= c.fin_cap) {
+    const bool wide = finish_wide_segment<K>(n, (K)0, KeyMax<K>::v, c.fin_cap);
     prof.begin(GGMS_KERNEL_OTHER, -1);
     k_tiny_setup<T><<<1, 1024, 0, st>>>(d_vec, n, nk, bufA, k_rank, k_state, (uint32_t*)P(bSmallStart),
                                         (uint32_t*)P(bSmallLen), (K*)P(bSmallKlo), (K*)P(bSmallKhi),
-                                        (uint32_t*)P(bSmallKStart), klist, ctl);
+                                        (uint32_t*)P(bSmallKStart), klist, ctl, wide ? 1u : 0u);
     prof.end();
     HIP_TRY(hipGetLastError());
     prof.begin(GGMS_KERNEL_FINISH, 0);
     k_finish<T><<<1, 1024, fin_lds, st>>>(
         bufA, ctl, (uint32_t*)P(bSmallStart), (uint32_t*)P(bSmallLen), (K*)P(bSmallKlo), (K*)P(bSmallKhi),
         (uint32_t*)P(bSmallKStart), nullptr, klist, k_rank, k_state, d_out, c.fin_cap, dflag);
+    if (wide)
+      k_finish_wide<T><<<1, 1024, 0, st>>>(bufA, ctl, (uint32_t*)P(bSmallStart), (uint32_t*)P(bSmallLen),
+                                           (K*)P(bSmallKlo), (K*)P(bSmallKhi), (uint32_t*)P(bSmallKStart), nullptr,
+                                           klist, k_rank, k_state, d_out, c.fin_cap);
     prof.end();
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(st));
@@ -576,6 +581,10 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
         k_finish<T><<<fin_grid, 1024, fin_lds, st>>>(dst, cp, pl.small_start, pl.small_len, pl.small_klo,
                                                      pl.small_khi, pl.small_kstart, pl.small_order, klist, k_rank,
                                                      k_state, d_out, c.fin_cap, dflag);
+        if (h.U_wide > 0)
+          k_finish_wide<T><<<std::min<uint32_t>(h.U_wide, (uint32_t)ctx->cus), 1024, 0, st>>>(
+              dst, cp, pl.small_start, pl.small_len, pl.small_klo, pl.small_khi, pl.small_kstart, pl.small_order,
+              klist, k_rank, k_state, d_out, c.fin_cap);
         prof.end();
         HIP_TRY(hipGetLastError());
       }

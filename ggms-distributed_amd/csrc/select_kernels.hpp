@@ -58,7 +58,8 @@ struct PassCtl {
   uint32_t nan_hi_bucket;  // bucket holding keys above key(+inf), or kNone32
   uint32_t append;         // first-pass compaction: next free position of the survivor buffer
   uint32_t l2_used;        // first pass: level-2 tables of the cell table in use
-  uint32_t pad[14];
+  uint32_t U_wide;         // small segments selected from global memory (the last U_wide of the finish order)
+  uint32_t pad[13];
 };
 
 struct Plan0Args {
@@ -227,9 +228,19 @@ __host__ __device__ __forceinline__ bool finish_fits_half(uint32_t len, K klo, K
   return len <= fin_cap / 2 || (uint64_t)(K)(khi - klo) <= (uint64_t)(H)~(H)0;
 }
 
-// Size classes of small segments for the finish order (1024 keys per class).
-constexpr uint32_t kOrderBins = 33;
-__host__ __device__ __forceinline__ uint32_t order_bin(uint32_t len) { return len >> 10 < kOrderBins - 1 ? len >> 10 : kOrderBins - 1; }
+// Size classes of small segments for the finish order (1024 keys per class,
+// the largest first), and class 0, ordered last, for the segments selected
+// from global memory.
+constexpr uint32_t kOrderBins = 34;
+template <class K>
+__host__ __device__ __forceinline__ bool finish_wide_segment(uint32_t len, K klo, K khi, uint32_t fin_cap) {
+  return len <= fin_cap && !finish_fits_half<K>(len, klo, khi, fin_cap);
+}
+template <class K>
+__host__ __device__ __forceinline__ uint32_t order_bin(uint32_t len, K klo, K khi, uint32_t fin_cap) {
+  if (finish_wide_segment<K>(len, klo, khi, fin_cap)) return 0;
+  return len >> 10 < kOrderBins - 2 ? 1 + (len >> 10) : kOrderBins - 1;
+}
 
 // words of a slot rank table (32 buckets per 64-bit word)
 __host__ __device__ __forceinline__ uint32_t rank_word_count(uint32_t B) { return (B + 31) / 32; }
@@ -1207,7 +1218,7 @@ __global__ void __launch_bounds__(1024) k_plan(PlanArgs<typename KeyTraits<T>::K
     uint32_t s = 0;
     for (uint32_t j = j0; j < j1; ++j) {
       s += kfill[j];
-      atomicAdd(&ord_cnt[order_bin(a.small_len[j])], 1u);
+      atomicAdd(&ord_cnt[order_bin<K>(a.small_len[j], a.small_klo[j], a.small_khi[j], a.fin_cap)], 1u);
     }
     uint32_t tot;
     uint32_t off = block_excl_scan_u32(s, scratch, &tot);
@@ -1221,6 +1232,7 @@ __global__ void __launch_bounds__(1024) k_plan(PlanArgs<typename KeyTraits<T>::K
   __threadfence_block();
   __syncthreads();
   if (t == 0) {  // largest size class first
+    sh_tot[5] = ord_cnt[0];
     uint32_t acc = 0;
     for (int c = (int)kOrderBins - 1; c >= 0; --c) {
       const uint32_t v = ord_cnt[c];
@@ -1244,7 +1256,7 @@ __global__ void __launch_bounds__(1024) k_plan(PlanArgs<typename KeyTraits<T>::K
   // finish order of the small segments: larger ones first, so the longest
   // workgroups start early instead of trailing the launch
   for (uint32_t j = t; j < U_small; j += nt)
-    a.small_order[atomicAdd(&ord_cnt[order_bin(a.small_len[j])], 1u)] = j;
+    a.small_order[atomicAdd(&ord_cnt[order_bin<K>(a.small_len[j], a.small_klo[j], a.small_khi[j], a.fin_cap)], 1u)] = j;
 
   // P9: next-pass tiling and first range of every tile
   uint32_t Gn = 0, tile = 0;
@@ -1280,6 +1292,7 @@ __global__ void __launch_bounds__(1024) k_plan(PlanArgs<typename KeyTraits<T>::K
     cc->N_big = N_big;
     cc->N_small = N_small;
     cc->live_ks = live_tot;
+    cc->U_wide = sh_tot[5];
     uint32_t nan = 0;
     if (ctl.nan_lo_bucket != kNone32) nan += a.totals[ctl.nan_lo_bucket];
     if (ctl.nan_hi_bucket != kNone32) nan += a.totals[ctl.nan_hi_bucket];
@@ -1804,9 +1817,10 @@ __device__ __forceinline__ void finish_segment(const C* __restrict__ s, uint32_t
   }
 }
 
-// Eight waves per SIMD (two workgroups per CU) caps the scalar registers: left
-// alone the compiler takes 91 SGPRs, which admits only one 1024-thread
-// workgroup per CU.  Dynamic LDS: fin_cap / 2 keys.
+// Eight waves per SIMD (two workgroups per CU) caps the registers at 64 VGPRs
+// (left alone the compiler takes 91 SGPRs, which admits only one 1024-thread
+// workgroup per CU).  The wide segments' global-memory path lives in its own
+// kernel: inlined here it made this one spill.  Dynamic LDS: fin_cap / 2 keys.
 template <class T>
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8))) k_finish(const typename KeyTraits<T>::K* __restrict__ buf, const PassCtl* __restrict__ ctl,
                                                  const uint32_t* __restrict__ small_start,
@@ -1832,20 +1846,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8)
     }
     const K klo = small_klo[j], khi = small_khi[j];
     const uint32_t q0 = small_kstart[j], q1 = small_kstart[j + 1];
-    if (!finish_fits_half<K>(len, klo, khi, fin_cap)) {
-      // wide and large: radix select from global memory, eight ks at a time
-      for (uint32_t c0 = q0; c0 < q1; c0 += kSelectMaxKs) {
-        const uint32_t c1 = q1 - c0 < kSelectMaxKs ? q1 : c0 + kSelectMaxKs;
-        uint32_t kk = 0, kr = 0;
-        if (threadIdx.x < c1 - c0) {
-          kk = klist[c0 + threadIdx.x];
-          kr = k_rank[kk];
-        }
-        finish_segment<T, K>(buf + st, len, klo, khi, (K)0, c0, c1, kk, kr, klist, k_rank, k_state, out, L);
-        __syncthreads();
-      }
-      continue;
-    }
+    if (!finish_fits_half<K>(len, klo, khi, fin_cap)) continue;  // k_finish_wide's
     // the ks and their ranks are fetched while the segment loads
     uint32_t kk = 0, kr = 0;
     if (threadIdx.x < q1 - q0 && q1 - q0 <= kSelectMaxKs) {
@@ -1865,6 +1866,43 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8)
   }
 }
 
+// Small segments too wide for half-width offsets and too large for full keys
+// in the finish kernel's LDS (rare: a few per call at most): radix select
+// straight from global memory, eight ks per sweep.  They are the last U_wide
+// entries of the finish order; one workgroup per segment.
+template <class T>
+__global__ void __launch_bounds__(1024) k_finish_wide(const typename KeyTraits<T>::K* __restrict__ buf, const PassCtl* __restrict__ ctl,
+                                                      const uint32_t* __restrict__ small_start,
+                                                      const uint32_t* __restrict__ small_len,
+                                                      const typename KeyTraits<T>::K* __restrict__ small_klo,
+                                                      const typename KeyTraits<T>::K* __restrict__ small_khi,
+                                                      const uint32_t* __restrict__ small_kstart,
+                                                      const uint32_t* __restrict__ small_order,
+                                                      const uint32_t* __restrict__ klist, uint32_t* __restrict__ k_rank,
+                                                      uint32_t* __restrict__ k_state, T* __restrict__ out,
+                                                      uint32_t fin_cap) {
+  using K = typename KeyTraits<T>::K;
+  __shared__ SelectLds L;
+  const uint32_t U_small = ctl->U_small, U_wide = ctl->U_wide;
+  for (uint32_t jj = U_small - U_wide + blockIdx.x; jj < U_small; jj += gridDim.x) {
+    const uint32_t j = small_order ? small_order[jj] : jj;
+    const uint32_t st = small_start[j], len = small_len[j];
+    const K klo = small_klo[j], khi = small_khi[j];
+    if (!finish_wide_segment<K>(len, klo, khi, fin_cap)) continue;  // k_finish's (or flagged there)
+    const uint32_t q0 = small_kstart[j], q1 = small_kstart[j + 1];
+    for (uint32_t c0 = q0; c0 < q1; c0 += kSelectMaxKs) {
+      const uint32_t c1 = q1 - c0 < kSelectMaxKs ? q1 : c0 + kSelectMaxKs;
+      uint32_t kk = 0, kr = 0;
+      if (threadIdx.x < c1 - c0) {
+        kk = klist[c0 + threadIdx.x];
+        kr = k_rank[kk];
+      }
+      finish_segment<T, K>(buf + st, len, klo, khi, (K)0, c0, c1, kk, kr, klist, k_rank, k_state, out, L);
+      __syncthreads();
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Tiny inputs (n <= finish capacity): one segment holding everything.
 // ---------------------------------------------------------------------------
@@ -1873,7 +1911,8 @@ __global__ void k_tiny_setup(const T* __restrict__ x, uint32_t n, uint32_t nk, t
                              uint32_t* __restrict__ k_rank, uint32_t* __restrict__ k_state,
                              uint32_t* __restrict__ small_start, uint32_t* __restrict__ small_len,
                              typename KeyTraits<T>::K* __restrict__ small_klo, typename KeyTraits<T>::K* __restrict__ small_khi,
-                             uint32_t* __restrict__ small_kstart, uint32_t* __restrict__ klist, PassCtl* ctl) {
+                             uint32_t* __restrict__ small_kstart, uint32_t* __restrict__ klist, PassCtl* ctl,
+                             uint32_t wide) {
   for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) buf[i] = KeyTraits<T>::enc(x[i]);
   __shared__ uint32_t scratch[32];
   uint32_t cnt = 0;
@@ -1896,6 +1935,7 @@ __global__ void k_tiny_setup(const T* __restrict__ x, uint32_t n, uint32_t nk, t
     ctl->U_big = 0;
     ctl->N_big = 0;
     ctl->U = 1;
+    ctl->U_wide = wide;
   }
 }
 

@@ -37,7 +37,7 @@ def test_library_exports_every_declared_symbol():
 def test_opts_defaults_and_status_names():
     import ggms
     o = ggms.default_opts()
-    assert (o.num_buckets, o.num_pivots, o.block_threads, o.blocks_per_cu, o.splitter) == (0, 17, 1024, 2, 0)
+    assert (o.num_buckets, o.num_pivots, o.block_threads, o.blocks_per_cu, o.splitter) == (0, 17, 1024, 0, 0)
     lib = ggms._lib.load()
     assert lib.ggms_status_str(0) == b"GGMS_OK" and lib.ggms_status_str(2) == b"GGMS_ENAN"
     ws = lib.ggms_workspace_bytes(0, 1 << 28, 1001, C.byref(o))
