@@ -131,7 +131,7 @@ __device__ __forceinline__ void batch_finish(BatchLds<typename KeyTraits<T>::K>&
     uint32_t M = pow2ceil_u32(len);
     for (uint32_t i = len + threadIdx.x; i < M; i += blockDim.x) L.A[i] = KeyMax<K>::v;
     __syncthreads();
-    bitonic_sort_lds<K>(L.A, M);
+    bitonic_sort_lds<K, true>(L.A, M);
     for (uint32_t j = j0 + threadIdx.x; j < j1; j += blockDim.x) out[L.Ko[j]] = KeyTraits<T>::dec(L.A[L.Kr[j] - 1]);
   }
   __syncthreads();
@@ -273,7 +273,7 @@ __global__ void __launch_bounds__(1024) k_batch(BatchArgs a) {
       if (threadIdx.x == 0) a.status[p] = GGMS_EINVAL;
       continue;
     }
-    bitonic_sort_lds<uint64_t>(kp, M);
+    bitonic_sort_lds<uint64_t, true>(kp, M);
     for (uint32_t j = threadIdx.x; j < nk; j += blockDim.x) {
       L.Kr[j] = (uint32_t)(kp[j] >> 32);
       L.Ko[j] = kb + (uint32_t)(kp[j] & 0xFFFFFFFFu);
@@ -334,7 +334,7 @@ __global__ void __launch_bounds__(1024) k_batch(BatchArgs a) {
       }
     }
     __syncthreads();
-    bitonic_sort_lds<K>(samp, Sp);
+    bitonic_sort_lds<K, true>(samp, Sp);
     if (threadIdx.x == 0) {
       // equal-quantile pivots, NaN / edge / one-key ranges as in the large-problem splitter
       uint32_t R = 0;
@@ -411,7 +411,7 @@ __global__ void __launch_bounds__(1024) k_batch(BatchArgs a) {
     for (uint32_t i = threadIdx.x; i < B0; i += blockDim.x) L.H[i] = 0;
     __syncthreads();
     constexpr int NV = Vec16<T>::N;
-    stream_tile_vec_any<T>(
+    stream_tile_vec_any<T, false, true>(
         x, 0, n,
         [&](const T (&v)[NV]) {
           K k[NV];
@@ -446,7 +446,7 @@ __global__ void __launch_bounds__(1024) k_batch(BatchArgs a) {
     batch_plan<T>(L, B0, 0, nk, 0, 0, interval0, out, stack, &s_nkids, slot, L.cursor, L.runb, scan_s);
     uint32_t nkids = s_nkids;
     if (nkids) {
-      stream_tile_vec_any<T>(
+      stream_tile_vec_any<T, false, true>(
           x, 0, n,
           [&](const T (&v)[NV]) {
             K k[NV];
@@ -514,7 +514,7 @@ __global__ void __launch_bounds__(1024) k_batch(BatchArgs a) {
           const uint32_t M = pow2ceil_u32(tot);
           for (uint32_t i = tot + threadIdx.x; i < M; i += blockDim.x) L.A[i] = KeyMax<K>::v;
           __syncthreads();
-          bitonic_sort_lds<K>(L.A, M);
+          bitonic_sort_lds<K, true>(L.A, M);
           for (uint32_t q = c; q < e; ++q) {
             const uint32_t off = c_reg[q] - c_reg[c];
             for (uint32_t j = c_j0[q] + threadIdx.x; j < c_j1[q]; j += blockDim.x)
@@ -552,7 +552,7 @@ __global__ void __launch_bounds__(1024) k_batch(BatchArgs a) {
         for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) L.H[i] = 0;
         __syncthreads();
         constexpr int NK = Vec16<K>::N;
-        stream_tile_vec_any<K>(
+        stream_tile_vec_any<K, false, true>(
             src, 0, it.len,
             [&](const K (&k)[NK]) {
               uint32_t b[NK];
@@ -576,7 +576,7 @@ __global__ void __launch_bounds__(1024) k_batch(BatchArgs a) {
             uint32_t pos = lds_claim(L.cursor, s, act);
             if (act) dst[pos] = key;
           };
-          stream_tile_vec_any<K>(
+          stream_tile_vec_any<K, false, true>(
               src, 0, it.len,
               [&](const K (&k)[NK]) {
 #pragma unroll

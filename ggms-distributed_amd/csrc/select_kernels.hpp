@@ -125,16 +125,33 @@ __device__ void block_incl_scan_lds(uint32_t* a, uint32_t L, uint32_t* scratch) 
 // [t*E, t*E + E): every compare-exchange distance below E runs in registers
 // (one load, all short-distance stages, one store), so only distances >= E
 // cost a barrier each.
+//
+// SW: between the first and the last register phase the array is kept
+// swizzled: inside every 128-byte block the eight 16-byte units are permuted
+// by XOR with the block number (keys keep their order inside a unit).  A
+// lane's E keys are E / (16 / sizeof(K)) whole units, read and written as
+// 16-byte accesses; the eight lanes served together by one 16-byte LDS access
+// then land on eight different units (plain, the 64-byte chunks of lanes t and
+// t + 2 share banks: 4-way conflicts for 16 words per lane), and the stages
+// whose lanes read consecutive keys touch each block's units once.  No extra
+// space: the permutation stays inside each block.
 template <class K>
+__device__ __forceinline__ uint32_t bitonic_swz(uint32_t i) {
+  constexpr uint32_t KU = 16 / sizeof(K), KB = 128 / sizeof(K);
+  return (i & ~(KB - 1)) | ((((i / KU) ^ (i / KB)) & 7u) * KU) | (i & (KU - 1));
+}
+
+template <class K, bool SW = false>
 __device__ __forceinline__ void bitonic_stage_lds(K* s, uint32_t M, uint32_t size, uint32_t stride) {
   for (uint32_t i = threadIdx.x; i < (M >> 1); i += blockDim.x) {
     uint32_t lo = 2 * i - (i & (stride - 1));
     uint32_t hi = lo + stride;
     bool asc = (lo & size) == 0;
-    K a = s[lo], b = s[hi];
+    const uint32_t plo = SW ? bitonic_swz<K>(lo) : lo, phi = SW ? bitonic_swz<K>(hi) : hi;
+    K a = s[plo], b = s[phi];
     if ((a > b) == asc) {
-      s[lo] = b;
-      s[hi] = a;
+      s[plo] = b;
+      s[phi] = a;
     }
   }
 }
@@ -158,9 +175,48 @@ __device__ __forceinline__ void bitonic_reg_merge(K (&r)[E], uint32_t base, uint
   }
 }
 
-// Threads past M / E hold no keys (they only take part in the barriers and
-// the LDS stages).
+// A lane's E keys in the swizzled layout: whole 16-byte units where E keys
+// fill at least one (the lane's first key then starts a unit), else key by key.
 template <class K, int E>
+__device__ __forceinline__ void swz_load(const K* s, uint32_t base, K (&r)[E]) {
+  constexpr int KU = 16 / sizeof(K);
+  if constexpr (E >= KU) {
+#pragma unroll
+    for (int j = 0; j < E / KU; ++j) {
+      const uint4 v = *reinterpret_cast<const uint4*>(s + bitonic_swz<K>(base + (uint32_t)(j * KU)));
+      K t[KU];
+      __builtin_memcpy(t, &v, 16);
+#pragma unroll
+      for (int q = 0; q < KU; ++q) r[j * KU + q] = t[q];
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < E; ++i) r[i] = s[bitonic_swz<K>(base + (uint32_t)i)];
+  }
+}
+template <class K, int E>
+__device__ __forceinline__ void swz_store(K* s, uint32_t base, const K (&r)[E]) {
+  constexpr int KU = 16 / sizeof(K);
+  if constexpr (E >= KU) {
+#pragma unroll
+    for (int j = 0; j < E / KU; ++j) {
+      K t[KU];
+#pragma unroll
+      for (int q = 0; q < KU; ++q) t[q] = r[j * KU + q];
+      uint4 v;
+      __builtin_memcpy(&v, t, 16);
+      *reinterpret_cast<uint4*>(s + bitonic_swz<K>(base + (uint32_t)(j * KU))) = v;
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < E; ++i) s[bitonic_swz<K>(base + (uint32_t)i)] = r[i];
+  }
+}
+
+// Threads past M / E hold no keys (they only take part in the barriers and
+// the LDS stages).  Input and output are in plain order.  SW needs s 16-byte
+// aligned and M a multiple of 128 bytes of keys.
+template <class K, int E, bool SW = false>
 __device__ void bitonic_sort_lds_reg(K* s, uint32_t M) {
   const uint32_t base = threadIdx.x * E;
   const bool own = base < M;
@@ -169,39 +225,60 @@ __device__ void bitonic_sort_lds_reg(K* s, uint32_t M) {
 #pragma unroll
     for (int i = 0; i < E; ++i) r[i] = s[base + i];
     for (uint32_t size = 2; size <= (uint32_t)E; size <<= 1) bitonic_reg_merge<K, E>(r, base, size, size >> 1);
+  }
+  if (SW) {
+    __syncthreads();  // every plain read done before the swizzled writes
+    if (own) swz_store<K, E>(s, base, r);
+  } else if (own) {
 #pragma unroll
     for (int i = 0; i < E; ++i) s[base + i] = r[i];
   }
   __syncthreads();
   for (uint32_t size = 2 * E; size <= M; size <<= 1) {
     for (uint32_t stride = size >> 1; stride >= (uint32_t)E; stride >>= 1) {
-      bitonic_stage_lds<K>(s, M, size, stride);
+      bitonic_stage_lds<K, SW>(s, M, size, stride);
       __syncthreads();
     }
     if (own) {
+      if (SW) {
+        swz_load<K, E>(s, base, r);
+      } else {
 #pragma unroll
-      for (int i = 0; i < E; ++i) r[i] = s[base + i];
+        for (int i = 0; i < E; ++i) r[i] = s[base + i];
+      }
       bitonic_reg_merge<K, E>(r, base, size, E / 2);
+    }
+    if (SW && size == M) {  // last merge: back to plain order
+      __syncthreads();
+      if (own) {
 #pragma unroll
-      for (int i = 0; i < E; ++i) s[base + i] = r[i];
+        for (int i = 0; i < E; ++i) s[base + i] = r[i];
+      }
+    } else if (own) {
+      if (SW) {
+        swz_store<K, E>(s, base, r);
+      } else {
+#pragma unroll
+        for (int i = 0; i < E; ++i) s[base + i] = r[i];
+      }
     }
     __syncthreads();
   }
 }
 
-template <class K>
+template <class K, bool SW = false>
 __device__ void bitonic_sort_lds(K* s, uint32_t M) {
   const uint32_t e = M / blockDim.x;
   if (M == e * blockDim.x &&
       (e == 4 || e == 8 || (e == 16 && sizeof(K) <= 4) || (e == 32 && sizeof(K) == 2))) {
-    if (e == 4) bitonic_sort_lds_reg<K, 4>(s, M);
-    else if (e == 8) bitonic_sort_lds_reg<K, 8>(s, M);
-    else if (e == 16) bitonic_sort_lds_reg<K, 16>(s, M);
-    else bitonic_sort_lds_reg<K, 32>(s, M);
+    if (e == 4) bitonic_sort_lds_reg<K, 4, SW>(s, M);
+    else if (e == 8) bitonic_sort_lds_reg<K, 8, SW>(s, M);
+    else if (e == 16) bitonic_sort_lds_reg<K, 16, SW>(s, M);
+    else bitonic_sort_lds_reg<K, 32, SW>(s, M);
     return;
   }
   if (e < 4 && M >= 256 && (M & (M - 1)) == 0) {  // fewer keys than 4 per thread: M / 4 threads hold 4 each
-    bitonic_sort_lds_reg<K, 4>(s, M);
+    bitonic_sort_lds_reg<K, 4, SW>(s, M);
     return;
   }
   for (uint32_t size = 2; size <= M; size <<= 1) {
@@ -337,14 +414,42 @@ __device__ __forceinline__ void stream_tile(const V* __restrict__ x, uint32_t t0
 // loads, for data read from HBM exactly once (the caller's vector in the first
 // pass).  Later passes read what the previous compaction just wrote, much of
 // it still in the last-level cache, and lose with the hint (pass-1 compaction
-// 0.151 ms plain, 0.160 ms with it).
-template <class V, bool NT = false, class FV, class F1>
+// 0.151 ms plain, 0.160 ms with it).  PIPE: the next four loads are issued
+// before the current four are processed (16 more VGPRs), for a workgroup that
+// streams alone on its CU and would otherwise leave HBM idle while it computes.
+template <class V, bool NT = false, bool PIPE = false, class FV, class F1>
 __device__ __forceinline__ void stream_tile_vec(const V* __restrict__ x, uint32_t t0, uint32_t t1, FV&& fv, F1&& f1) {
   constexpr int N = Vec16<V>::N;
   const uint32_t nt = blockDim.x;
   const uint4* xv = reinterpret_cast<const uint4*>(x);
   uint32_t v1 = t1 / N;
   uint32_t v = t0 / N + threadIdx.x;
+  if constexpr (PIPE) {
+    if (v + 3 * nt < v1) {
+      uint4 q[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) q[u] = xv[v + u * nt];
+      for (;;) {
+        const uint32_t vn = v + 4 * nt;
+        const bool more = vn + 3 * nt < v1;
+        uint4 r[4];
+        if (more) {
+#pragma unroll
+          for (int u = 0; u < 4; ++u) r[u] = xv[vn + u * nt];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          V e[N];
+          Vec16<V>::unpack(q[u], e);
+          fv(e);
+        }
+        v = vn;
+        if (!more) break;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) q[u] = r[u];
+      }
+    }
+  }
   for (; v + 3 * nt < v1; v += 4 * nt) {
     uint4 q0 = NT ? ld_stream(xv + v) : xv[v];
     uint4 q1 = NT ? ld_stream(xv + v + nt) : xv[v + nt];
@@ -372,7 +477,7 @@ __device__ __forceinline__ void stream_tile_vec(const V* __restrict__ x, uint32_
 
 // Any start: the elements before the first 16-byte boundary go one per lane,
 // the rest through the vector path.
-template <class V, bool NT = false, class FV, class F1>
+template <class V, bool NT = false, bool PIPE = false, class FV, class F1>
 __device__ __forceinline__ void stream_tile_vec_any(const V* __restrict__ x, uint32_t t0, uint32_t t1, FV&& fv,
                                                     F1&& f1) {
   if (t0 >= t1) return;
@@ -384,7 +489,7 @@ __device__ __forceinline__ void stream_tile_vec_any(const V* __restrict__ x, uin
   uint32_t head = mis ? (16u - mis) / (uint32_t)sizeof(V) : 0u;
   if (head > t1 - t0) head = t1 - t0;
   if (threadIdx.x < head) f1(x[t0 + threadIdx.x]);
-  stream_tile_vec<V, NT>(x + t0 + head, 0, t1 - t0 - head, fv, f1);
+  stream_tile_vec<V, NT, PIPE>(x + t0 + head, 0, t1 - t0 - head, fv, f1);
 }
 
 // hist[b[j]] += 1 for N buckets per lane (every lane of the wave active for
@@ -1808,7 +1913,7 @@ __device__ __forceinline__ void finish_segment(const C* __restrict__ s, uint32_t
     const uint32_t M = pow2ceil_u32(len);
     for (uint32_t i = len + threadIdx.x; i < M; i += blockDim.x) w[i] = (C)~(C)0;
     __syncthreads();
-    bitonic_sort_lds<C>(w, M);
+    bitonic_sort_lds<C, sizeof(C) <= 4>(w, M);  // 8-byte keys: the swizzle costs the 64-VGPR budget
     for (uint32_t qq = q0 + threadIdx.x; qq < q1; qq += blockDim.x) {
       const uint32_t k = klist[qq];
       out[k] = KeyTraits<T>::dec((K)(base + (K)s[k_rank[k] - 1]));

@@ -53,7 +53,7 @@ def gen_dir(tmp_path_factory):
 
 def _build_and_run(gen_dir, driver, args=(), timeout=900):
     exe = os.path.join(gen_dir, driver.replace(".cpp", ""))
-    cmd = ["g++", "-std=c++20", "-O2", "-pthread", "-I", EMU, "-I", os.path.join(ROOT, "include"), "-I", gen_dir,
+    cmd = ["g++", "-std=c++20", "-O2", "-fno-strict-aliasing", "-pthread", "-I", EMU, "-I", os.path.join(ROOT, "include"), "-I", gen_dir,
            os.path.join(EMU, driver), "-o", exe]
     r = subprocess.run(cmd, capture_output=True, text=True)
     assert r.returncode == 0, r.stderr[-4000:]
