@@ -131,7 +131,7 @@ __device__ __forceinline__ void batch_finish(BatchLds<typename KeyTraits<T>::K>&
     uint32_t M = pow2ceil_u32(len);
     for (uint32_t i = len + threadIdx.x; i < M; i += blockDim.x) L.A[i] = KeyMax<K>::v;
     __syncthreads();
-    bitonic_sort_lds<K, true>(L.A, M);
+    bitonic_sort_lds<K, true, false>(L.A, M);
     for (uint32_t j = j0 + threadIdx.x; j < j1; j += blockDim.x) out[L.Ko[j]] = KeyTraits<T>::dec(L.A[L.Kr[j] - 1]);
   }
   __syncthreads();
@@ -273,7 +273,7 @@ __global__ void __launch_bounds__(1024) k_batch(BatchArgs a) {
       if (threadIdx.x == 0) a.status[p] = GGMS_EINVAL;
       continue;
     }
-    bitonic_sort_lds<uint64_t, true>(kp, M);
+    bitonic_sort_lds<uint64_t, true, false>(kp, M);
     for (uint32_t j = threadIdx.x; j < nk; j += blockDim.x) {
       L.Kr[j] = (uint32_t)(kp[j] >> 32);
       L.Ko[j] = kb + (uint32_t)(kp[j] & 0xFFFFFFFFu);
@@ -334,7 +334,7 @@ __global__ void __launch_bounds__(1024) k_batch(BatchArgs a) {
       }
     }
     __syncthreads();
-    bitonic_sort_lds<K, true>(samp, Sp);
+    bitonic_sort_lds<K, true, false>(samp, Sp);
     if (threadIdx.x == 0) {
       // equal-quantile pivots, NaN / edge / one-key ranges as in the large-problem splitter
       uint32_t R = 0;
@@ -514,7 +514,7 @@ __global__ void __launch_bounds__(1024) k_batch(BatchArgs a) {
           const uint32_t M = pow2ceil_u32(tot);
           for (uint32_t i = tot + threadIdx.x; i < M; i += blockDim.x) L.A[i] = KeyMax<K>::v;
           __syncthreads();
-          bitonic_sort_lds<K, true>(L.A, M);
+          bitonic_sort_lds<K, true, false>(L.A, M);
           for (uint32_t q = c; q < e; ++q) {
             const uint32_t off = c_reg[q] - c_reg[c];
             for (uint32_t j = c_j0[q] + threadIdx.x; j < c_j1[q]; j += blockDim.x)

@@ -266,19 +266,92 @@ __device__ void bitonic_sort_lds_reg(K* s, uint32_t M) {
   }
 }
 
-template <class K, bool SW = false>
+// Key exchange with lane (lane ^ j) of the same wave (j < 64).
+template <class K>
+__device__ __forceinline__ K shfl_xor_key(K v, int j) {
+  if constexpr (sizeof(K) == 8) {
+    const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)v, j, 64);
+    const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(v >> 32), j, 64);
+    return ((K)hi << 32) | (K)lo;
+  } else {
+    return (K)(uint32_t)__shfl_xor((int)(uint32_t)v, j, 64);
+  }
+}
+
+// Bitonic sort with the keys in registers throughout: thread t holds keys
+// [t*E, t*E + E) of M = E * T, T = M / E owning threads (a multiple of 64).  A
+// compare-exchange at distance d pairs thread t with t ^ (d / E): d < E stays
+// in registers, d / E < 64 is a lane shuffle inside the wave (no barrier),
+// and only d >= 64 E goes through LDS (own keys out, partner's keys in, two
+// barriers).  For M = 8192, E = 8 that is 10 barrier stages instead of 55.
+template <class K, int E>
+__device__ void bitonic_sort_shfl(K* s, uint32_t M) {
+  const uint32_t t = threadIdx.x, base = t * E;
+  const bool own = base < M;  // uniform per wave: M / E is a multiple of 64
+  K r[E];
+  if (own) {
+#pragma unroll
+    for (int i = 0; i < E; ++i) r[i] = s[base + i];
+    for (uint32_t size = 2; size <= (uint32_t)E; size <<= 1) bitonic_reg_merge<K, E>(r, base, size, size >> 1);
+  }
+  for (uint32_t size = 2 * E; size <= M; size <<= 1) {
+    const bool asc = (base & size) == 0;
+    for (uint32_t stride = size >> 1; stride >= (uint32_t)E; stride >>= 1) {
+      const uint32_t j = stride / E;
+      const bool keep_min = ((t & j) == 0) == asc;
+      if (j >= 64) {
+        if (own) {
+#pragma unroll
+          for (int i = 0; i < E; ++i) s[base + i] = r[i];
+        }
+        __syncthreads();
+        K p[E];
+        if (own) {
+#pragma unroll
+          for (int i = 0; i < E; ++i) p[i] = s[(t ^ j) * E + i];
+        }
+        __syncthreads();  // every partner read done before the next stage writes
+        if (own) {
+#pragma unroll
+          for (int i = 0; i < E; ++i) r[i] = keep_min ? (p[i] < r[i] ? p[i] : r[i]) : (p[i] > r[i] ? p[i] : r[i]);
+        }
+      } else if (own) {
+#pragma unroll
+        for (int i = 0; i < E; ++i) {
+          const K o = shfl_xor_key<K>(r[i], (int)j);
+          r[i] = keep_min ? (o < r[i] ? o : r[i]) : (o > r[i] ? o : r[i]);
+        }
+      }
+    }
+    if (own) bitonic_reg_merge<K, E>(r, base, size, E / 2);
+  }
+  if (own) {
+#pragma unroll
+    for (int i = 0; i < E; ++i) s[base + i] = r[i];
+  }
+  __syncthreads();
+}
+
+// SHFL: the register-resident form (bitonic_sort_shfl) where it applies; it
+// holds 2 E keys per thread at its LDS stages, so kernels held to 64 VGPRs
+// pass false and keep the LDS-stage form.
+template <class K, bool SW = false, bool SHFL = true>
 __device__ void bitonic_sort_lds(K* s, uint32_t M) {
   const uint32_t e = M / blockDim.x;
   if (M == e * blockDim.x &&
       (e == 4 || e == 8 || (e == 16 && sizeof(K) <= 4) || (e == 32 && sizeof(K) == 2))) {
-    if (e == 4) bitonic_sort_lds_reg<K, 4, SW>(s, M);
+    if (SHFL && e == 4) bitonic_sort_shfl<K, 4>(s, M);
+    else if (SHFL && e == 8) bitonic_sort_shfl<K, 8>(s, M);
+    else if (SHFL && e == 16) bitonic_sort_shfl<K, 16>(s, M);
+    else if (e == 4) bitonic_sort_lds_reg<K, 4, SW>(s, M);
     else if (e == 8) bitonic_sort_lds_reg<K, 8, SW>(s, M);
     else if (e == 16) bitonic_sort_lds_reg<K, 16, SW>(s, M);
     else bitonic_sort_lds_reg<K, 32, SW>(s, M);
     return;
   }
   if (e < 4 && M >= 256 && (M & (M - 1)) == 0) {  // fewer keys than 4 per thread: M / 4 threads hold 4 each
-    bitonic_sort_lds_reg<K, 4, SW>(s, M);
+    if (SHFL) bitonic_sort_shfl<K, 4>(s, M);
+    else bitonic_sort_lds_reg<K, 4, SW>(s, M);
     return;
   }
   for (uint32_t size = 2; size <= M; size <<= 1) {
@@ -1913,7 +1986,7 @@ __device__ __forceinline__ void finish_segment(const C* __restrict__ s, uint32_t
     const uint32_t M = pow2ceil_u32(len);
     for (uint32_t i = len + threadIdx.x; i < M; i += blockDim.x) w[i] = (C)~(C)0;
     __syncthreads();
-    bitonic_sort_lds<C, sizeof(C) <= 4>(w, M);  // 8-byte keys: the swizzle costs the 64-VGPR budget
+    bitonic_sort_lds<C, false, false>(w, M);  // within the 64-VGPR budget: plain LDS stages
     for (uint32_t qq = q0 + threadIdx.x; qq < q1; qq += blockDim.x) {
       const uint32_t k = klist[qq];
       out[k] = KeyTraits<T>::dec((K)(base + (K)s[k_rank[k] - 1]));

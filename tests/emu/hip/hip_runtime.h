@@ -88,6 +88,27 @@ inline uint32_t __builtin_amdgcn_mbcnt_lo(uint32_t, uint32_t x) { return x; }
 inline uint32_t __builtin_amdgcn_mbcnt_hi(uint32_t, uint32_t x) { return x; }
 inline float __expf(float x) { return std::exp(x); }
 float __shfl_xor(float v, int mask, int width);
+// integer lane exchange inside a wave (all 64 threads of the wave must call)
+struct EmuWaveX { std::mutex m; std::condition_variable cv; uint32_t arrived = 0, gen = 0; int v[64]; int out[64]; };
+inline EmuWaveX g_emu_wx[16];
+inline int __shfl_xor(int v, int mask, int width) {
+  (void)width;
+  const uint32_t w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  EmuWaveX& s = g_emu_wx[w];
+  std::unique_lock<std::mutex> lk(s.m);
+  const uint32_t g = s.gen;
+  s.v[l] = v;
+  if (++s.arrived == 64) {
+    for (int i = 0; i < 64; ++i) s.out[i] = s.v[i ^ (mask & 63)];
+    s.arrived = 0;
+    s.gen++;
+    s.cv.notify_all();
+  } else if (!s.cv.wait_for(lk, std::chrono::seconds(20), [&] { return s.gen != g; })) {
+    fprintf(stderr, "EMU: shuffle timeout (thread %u)\n", threadIdx.x);
+    std::abort();
+  }
+  return s.out[l];
+}
 inline unsigned long long wall_clock64() { return 0; }
 extern unsigned char* g_dyn_smem;
 
