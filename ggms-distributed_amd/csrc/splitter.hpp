@@ -155,6 +155,7 @@ __global__ void __launch_bounds__(1024) k_plan0(const T* __restrict__ x, Plan0Ar
   //    first use (random reads over the whole vector: each one is a full
   //    memory round trip, so they must overlap)
   constexpr uint32_t kSampleUnroll = 8;
+  constexpr uint32_t kSampleRun = 16;
   for (uint32_t i0 = threadIdx.x; i0 < Spad; i0 += kSampleUnroll * blockDim.x) {
     uint32_t pos[kSampleUnroll];
 #pragma unroll
@@ -168,6 +169,14 @@ __global__ void __launch_bounds__(1024) k_plan0(const T* __restrict__ x, Plan0Ar
           uint32_t j = (uint32_t)(((uint64_t)philox_draw(a.seed, kStreamSubSample, i).v[0] * S1) >> 32);
           uint64_t pp = (uint64_t)j * stride;
           p = pp < n ? (uint32_t)pp : n - 1;
+        } else if (a.mode == kModeLearned || a.mode == kModeSketch) {
+          // runs of kSampleRun consecutive elements from random starts: one
+          // 64-byte read (and one address translation) per run instead of per
+          // element; the linear form keeps independent positions
+          const uint32_t c = i / kSampleRun;
+          const uint32_t start =
+              (uint32_t)(((uint64_t)philox_draw(a.seed, kStreamSample, c).v[0] * (n - kSampleRun + 1)) >> 32);
+          p = start + i % kSampleRun;
         } else {
           p = (uint32_t)(((uint64_t)philox_draw(a.seed, kStreamSample, i).v[0] * n) >> 32);
         }
