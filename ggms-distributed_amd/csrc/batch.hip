@@ -41,6 +41,9 @@ constexpr uint32_t kBatchNB = 8192;        // LDS histogram capacity
 constexpr uint32_t kBatchMaxPieces = 64;
 constexpr uint32_t kBatchMaxRanges = 2 * (kBatchMaxPieces + 1) + 6;
 constexpr uint32_t kBatchClasses = GGMS_BATCH_CLASSES;
+// a run of first-pass children whose every member has at most this many ks is
+// answered by per-wave radix selects instead of one sort of the whole run
+constexpr uint32_t kWaveSelectMaxKs = 8;
 
 template <class K> struct BatchItem {
   K klo, khi;
@@ -224,6 +227,7 @@ __global__ void __launch_bounds__(1024) k_batch(BatchArgs a) {
   __shared__ uint32_t hist256[256];
   __shared__ uint32_t bcast[2];
   __shared__ uint32_t scan_s[32];
+  __shared__ uint32_t s_wbc[16][2];  // per-wave digit broadcast of wave_radix_select
   BatchLds<K> L;
   L.carve(smem);
   BatchItem<K>* stack = reinterpret_cast<BatchItem<K>*>(a.stacks) + (size_t)blockIdx.x * a.stack_cap;
@@ -474,20 +478,37 @@ __global__ void __launch_bounds__(1024) k_batch(BatchArgs a) {
     }
     // ---- first-pass children.  They sit in sA in bucket order, so their key
     // intervals are disjoint and increasing: a run of consecutive small
-    // children is loaded and sorted as one batch and each k is read off at
-    // (child offset in the batch + rank in child).  Big children move to the
-    // front of the stack for re-bucketing.
+    // children is loaded as one batch; each k is then either radix-selected
+    // inside its child by one wave (every child of the run holds at most
+    // kWaveSelectMaxKs ks: the waves work on different children with no
+    // workgroup barrier) or read off one sort of the whole run at (child
+    // offset in the batch + rank in child).  Big children move to the front
+    // of the stack for re-bucketing.
     {
+      // child tables in the histogram's space (nkids <= kBatchMaxKs: 16 bytes
+      // per child always fit), key intervals behind them when they fit too;
+      // the wave histograms take the compaction cursors' and the k-bucket
+      // table's space (both free until the work stack)
       uint32_t* c_len = L.H;
-      uint32_t* c_j0 = L.H + kBatchMaxKs;
-      uint32_t* c_j1 = L.H + 2 * kBatchMaxKs;
-      uint32_t* c_reg = L.H + 3 * kBatchMaxKs;
+      uint32_t* c_j0 = c_len + nkids;
+      uint32_t* c_j1 = c_j0 + nkids;
+      uint32_t* c_reg = c_j1 + nkids;
+      K* c_klo = reinterpret_cast<K*>(c_reg + nkids);
+      K* c_khi = c_klo + nkids;
+      const bool kfit = (size_t)nkids * (16 + 2 * sizeof(K)) <= (size_t)kBatchNB * 4;
+      uint32_t* whist = L.cursor;  // 256 bins per wave
+      static_assert(16 * kBatchMaxKs <= kBatchNB * 4, "child tables");
+      static_assert(16 * 256 <= 2 * kBatchMaxKs, "wave histograms: cursor + runb");
       for (uint32_t c = threadIdx.x; c < nkids; c += blockDim.x) {
         const BatchItem<K> it = stack[c];
         c_len[c] = it.len;
         c_j0[c] = it.j0;
         c_j1[c] = it.j1;
         c_reg[c] = it.region;
+        if (kfit) {
+          c_klo[c] = it.klo;
+          c_khi[c] = it.khi;
+        }
       }
       __syncthreads();
       if (threadIdx.x == 0) {
@@ -504,12 +525,26 @@ __global__ void __launch_bounds__(1024) k_batch(BatchArgs a) {
           continue;
         }
         uint32_t e = c, tot = 0;
-        while (e < nkids && c_len[e] <= FIN && tot + c_len[e] <= FIN) tot += c_len[e++];
+        bool few = kfit;
+        while (e < nkids && c_len[e] <= FIN && tot + c_len[e] <= FIN) {
+          few &= c_j1[e] - c_j0[e] <= kWaveSelectMaxKs;
+          tot += c_len[e++];
+        }
         const K* src = sA + c_reg[c];
         for (uint32_t i = threadIdx.x; i < tot; i += blockDim.x) L.A[i] = src[i];
         __syncthreads();
         if (e - c == 1) {
           batch_finish<T>(L, tot, (K)0, KeyMax<K>::v, c_j0[c], c_j1[c], out, hist256, bcast);
+        } else if (few) {
+          const uint32_t w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+          for (uint32_t q = c + w; q < e; q += nw) {
+            const K* cs = L.A + (c_reg[q] - c_reg[c]);
+            for (uint32_t j = c_j0[q]; j < c_j1[q]; ++j) {
+              const K key = wave_radix_select<K>(cs, c_len[q], c_klo[q], c_khi[q], L.Kr[j], whist + w * 256, s_wbc[w]);
+              if (lane_id() == 0) out[L.Ko[j]] = KeyTraits<T>::dec(key);
+            }
+          }
+          __syncthreads();
         } else {
           const uint32_t M = pow2ceil_u32(tot);
           for (uint32_t i = tot + threadIdx.x; i < M; i += blockDim.x) L.A[i] = KeyMax<K>::v;

@@ -150,6 +150,14 @@ __device__ __forceinline__ uint32_t mask_below(uint64_t m) {
   return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 __device__ __forceinline__ uint64_t wave_ballot(bool p) { return __ballot(p); }
+// Orders the LDS accesses of the lanes of one wave around this point (the
+// wave's LDS instructions execute in order; this keeps the compiler from
+// moving them across it and waits for the ones issued).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 __device__ __forceinline__ uint32_t readlane_u32(uint32_t v, uint32_t lane) {
   return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)lane);
 }

@@ -86,6 +86,47 @@ __device__ __forceinline__ uint32_t wave_incl_scan_u32(uint32_t v) {
   return v;
 }
 
+// Key of rank `rank` (1-based) among a[0, len), all keys in [klo, khi], by
+// ONE wave with no workgroup barrier: 8-bit digits over the bits where klo
+// and khi differ, a 256-bin histogram in the wave's own LDS (hist), the digit
+// found by a wave scan and handed to every lane through bc[0..1].
+template <class K>
+__device__ K wave_radix_select(const K* __restrict__ a, uint32_t len, K klo, K khi, uint32_t rank, uint32_t* hist,
+                               uint32_t* bc) {
+  constexpr uint32_t kb = sizeof(K) * 8;
+  const uint32_t l = lane_id();
+  const uint32_t vary = bitlen64((uint64_t)(klo ^ khi));
+  K pre = vary >= kb ? (K)0 : (K)(klo & (K)(~(K)0 << vary));
+  for (int shift = (int)((vary + 7) & ~7u) - 8; shift >= 0; shift -= 8) {
+    const K hmask = (uint32_t)shift + 8 >= kb ? (K)0 : (K)(~(K)0 << (shift + 8));
+    const K want = (K)(pre & hmask);
+    for (uint32_t b = l; b < 256; b += 64) hist[b] = 0;
+    wave_sync();
+    for (uint32_t i = l; i < len; i += 64) {
+      const K key = a[i];
+      if ((K)(key & hmask) == want) atomicAdd(&hist[(uint32_t)(key >> shift) & 255u], 1u);
+    }
+    wave_sync();
+    const uint32_t c0 = hist[4 * l], c1 = hist[4 * l + 1], c2 = hist[4 * l + 2], c3 = hist[4 * l + 3];
+    const uint32_t sum = c0 + c1 + c2 + c3;
+    uint32_t cum = wave_incl_scan_u32(sum) - sum;
+    const uint32_t c[4] = {c0, c1, c2, c3};
+#pragma unroll
+    for (uint32_t b = 0; b < 4; ++b) {
+      if (cum < rank && cum + c[b] >= rank) {
+        bc[0] = 4 * l + b;
+        bc[1] = rank - cum;
+      }
+      cum += c[b];
+    }
+    wave_sync();
+    pre = (K)(pre | ((K)bc[0] << shift));
+    rank = bc[1];
+    wave_sync();  // every lane has read bc and hist before the next round writes them
+  }
+  return pre;
+}
+
 // Exclusive scan of one value per thread over the whole workgroup.
 // scratch: >= 16 u32 of LDS.  Returns exclusive prefix, total in *total.
 __device__ __forceinline__ uint32_t block_excl_scan_u32(uint32_t v, uint32_t* scratch, uint32_t* total) {

@@ -114,3 +114,20 @@ extern unsigned char* g_dyn_smem;
 
 // wave-wide inclusive scan among the 64 threads of a wave (all must call)
 uint32_t emu_wave_incl_scan(uint32_t v);
+
+// wave-level ordering point: a barrier among the 64 threads of the caller's wave
+#define __builtin_amdgcn_fence(order, scope) ((void)0)
+inline EmuWaveX g_emu_wb[16];
+inline void __builtin_amdgcn_wave_barrier() {
+  EmuWaveX& s = g_emu_wb[threadIdx.x >> 6];
+  std::unique_lock<std::mutex> lk(s.m);
+  const uint32_t g = s.gen;
+  if (++s.arrived == 64) {
+    s.arrived = 0;
+    s.gen++;
+    s.cv.notify_all();
+  } else if (!s.cv.wait_for(lk, std::chrono::seconds(20), [&] { return s.gen != g; })) {
+    fprintf(stderr, "EMU: wave barrier timeout (thread %u)\n", threadIdx.x);
+    std::abort();
+  }
+}

@@ -60,6 +60,53 @@ template <class K> int run(uint64_t seed, uint32_t nt) {
     }
   return bad ? 1 : 0;
 }
+// wave_radix_select: every wave of the workgroup selects its own rank in its
+// own segment (no workgroup barrier in between), against std::sort
+template <class K> int run_wave(uint64_t seed, uint32_t nt) {
+  std::mt19937_64 rng(seed);
+  const uint32_t nw = nt / 64;
+  std::vector<std::vector<K>> seg(nw);
+  std::vector<K> lo(nw), hi(nw), want(nw), got(nw);
+  std::vector<uint32_t> rank(nw);
+  for (uint32_t w = 0; w < nw; ++w) {
+    const uint32_t len = 1 + (uint32_t)(rng() % 700);
+    const uint32_t bits = 1 + (uint32_t)(rng() % (sizeof(K) * 8));
+    const K wd = bits >= sizeof(K) * 8 ? (K)~(K)0 : (K)(((K)1 << bits) - 1);
+    K klo = wd == (K)~(K)0 ? (K)0 : (K)rng();
+    if ((K)(klo + wd) < klo) klo = (K)(klo - wd);  // keep [klo, klo + wd] unwrapped
+    const K khi = (K)(klo + wd);
+    seg[w].resize(len);
+    for (auto& v : seg[w]) v = (K)(klo + (K)(wd == (K)~(K)0 ? rng() : rng() % ((uint64_t)wd + 1)));
+    if (rng() & 1) for (auto& v : seg[w]) if (rng() % 3 == 0) v = seg[w][0];  // duplicates
+    std::vector<K> srt = seg[w];
+    std::sort(srt.begin(), srt.end());
+    rank[w] = 1 + (uint32_t)(rng() % len);
+    want[w] = srt[rank[w] - 1];
+    lo[w] = klo;
+    hi[w] = khi;
+  }
+  static uint32_t hist[16 * 256];
+  static uint32_t bc[16][2];
+  blockDim = dim3(nt); g_bar.reset(nt);
+  std::vector<std::thread> th;
+  for (uint32_t t = 0; t < nt; ++t)
+    th.emplace_back([&, t] {
+      threadIdx = dim3(t);
+      const uint32_t w = t >> 6;
+      const K k = ggms::wave_radix_select<K>(seg[w].data(), (uint32_t)seg[w].size(), lo[w], hi[w], rank[w], hist + 256 * w, bc[w]);
+      if ((t & 63) == 0) got[w] = k;
+      g_bar.exit_thread();
+    });
+  for (auto& x : th) x.join();
+  int bad = 0;
+  for (uint32_t w = 0; w < nw; ++w)
+    if (got[w] != want[w]) {
+      if (bad < 3) printf("WAVE FAIL K=%zu len=%zu rank=%u got %llx want %llx\n", sizeof(K), seg[w].size(), rank[w],
+                          (unsigned long long)got[w], (unsigned long long)want[w]);
+      ++bad;
+    }
+  return bad ? 1 : 0;
+}
 int main(int argc, char** argv) {
   const int trials = argc > 1 ? atoi(argv[1]) : 60;
   int bad = 0;
@@ -67,6 +114,8 @@ int main(int argc, char** argv) {
     bad += run<uint32_t>(1000 + i, (i & 1) ? 256 : 128);
     bad += run<uint64_t>(5000 + i, (i & 1) ? 128 : 256);
     bad += run<uint16_t>(9000 + i, (i & 1) ? 256 : 128);
+    bad += run_wave<uint32_t>(13000 + i, 256);
+    bad += run_wave<uint64_t>(17000 + i, 128);
   }
   printf(bad ? "select FAIL\n" : "select ok\n");
   return bad;

@@ -69,7 +69,8 @@ def test_lds_bitonic_sort_network(gen_dir):
 
 
 def test_finish_multi_rank_radix_select(gen_dir):
-    """Up to eight ranks of one segment in a single radix sweep per digit."""
+    """Up to eight ranks of one segment in a single radix sweep per digit; the one-wave radix
+    select (every wave of the workgroup on its own segment and rank)."""
     out = _build_and_run(gen_dir, "select_driver.cpp", [60])
     assert "select ok" in out
 
