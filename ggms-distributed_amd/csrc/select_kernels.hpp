@@ -154,6 +154,37 @@ __device__ __forceinline__ uint32_t block_excl_scan_u32(uint32_t v, uint32_t* sc
   return off + inc - v;
 }
 
+// Four exclusive scans at once (one pair of barriers instead of four).
+// scratch: >= 64 u32 of LDS.
+__device__ __forceinline__ void block_excl_scan_u32x4(const uint32_t (&v)[4], uint32_t (&off)[4], uint32_t (&tot)[4],
+                                                      uint32_t* scratch) {
+  const uint32_t lane = lane_id(), wv = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+  uint32_t inc[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) inc[i] = wave_incl_scan_u32(v[i]);
+  __syncthreads();
+  if (lane == 63) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) scratch[wv * 4 + i] = inc[i];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    off[i] = 0;
+    tot[i] = 0;
+  }
+  for (uint32_t w = 0; w < nw; ++w) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const uint32_t s = scratch[w * 4 + i];
+      if (w < wv) off[i] += s;
+      tot[i] += s;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) off[i] += inc[i] - v[i];
+}
+
 // In-place inclusive scan of an LDS array of length L.  scratch >= 16 u32.
 __device__ void block_incl_scan_lds(uint32_t* a, uint32_t L, uint32_t* scratch) {
   const uint32_t nt = blockDim.x, t = threadIdx.x;
@@ -1225,7 +1256,7 @@ template <class T>
 __global__ void __launch_bounds__(1024) k_plan(PlanArgs<typename KeyTraits<T>::K> a) {
   using K = typename KeyTraits<T>::K;
   extern __shared__ __align__(16) unsigned char smem[];
-  __shared__ uint32_t scratch[32];
+  __shared__ uint32_t scratch[64];
   __shared__ uint32_t sh_tot[8];
   const uint32_t nt = blockDim.x, t = threadIdx.x;
   const PassCtl& ctl = *a.ctl;
@@ -1325,11 +1356,11 @@ __global__ void __launch_bounds__(1024) k_plan(PlanArgs<typename KeyTraits<T>::K
       if (cls == kClsBig) { nbig++; cbig += c; }
       else if (cls == kClsSmall) { nsml++; csml += c; }
     }
-    uint32_t T0, T1, T2, T3;
-    uint32_t e0 = block_excl_scan_u32(nbig, scratch, &T0);
-    uint32_t e1 = block_excl_scan_u32(cbig, scratch, &T1);
-    uint32_t e2 = block_excl_scan_u32(nsml, scratch, &T2);
-    uint32_t e3 = block_excl_scan_u32(csml, scratch, &T3);
+    const uint32_t sv[4] = {nbig, cbig, nsml, csml};
+    uint32_t se[4], sT[4];
+    block_excl_scan_u32x4(sv, se, sT, scratch);
+    uint32_t e0 = se[0], e1 = se[1], e2 = se[2], e3 = se[3];
+    const uint32_t T0 = sT[0], T1 = sT[1], T2 = sT[2], T3 = sT[3];
     for (uint32_t q = q0; q < q1; ++q) {
       uint32_t cls = a.act_cls[q], c = a.act_cnt[q];
       if (cls == kClsBig) {
