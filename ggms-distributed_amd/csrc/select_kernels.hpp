@@ -60,7 +60,8 @@ struct PassCtl {
   uint32_t l2_used;        // first pass: level-2 tables of the cell table in use
   uint32_t U_wide;         // small segments selected from global memory (the last U_wide of the finish order)
   uint32_t U_wave;         // small segments finished one per wave (the U_wave before those)
-  uint32_t pad[12];
+  uint32_t small_max;      // longest small segment of the pass
+  uint32_t pad[11];
 };
 
 struct Plan0Args {
@@ -1269,6 +1270,7 @@ __global__ void __launch_bounds__(1024) k_plan(PlanArgs<typename KeyTraits<T>::K
 
   __shared__ uint32_t ord_cnt[kOrderBins];  // small segments per size class (finish order)
   if (t < kOrderBins) ord_cnt[t] = 0;
+  if (t == 0) sh_tot[7] = 0;  // longest small segment
   // P0: histogram prefix, reset tables
   if (a.rank_words)
     for (uint32_t w = t; w < rank_word_count(B); w += nt) {
@@ -1388,6 +1390,7 @@ __global__ void __launch_bounds__(1024) k_plan(PlanArgs<typename KeyTraits<T>::K
   if (Bn > a.B_cap) Bn = a.B_cap;
 
   // P5: slot tables, small segment descriptors, next ranges (unplaced)
+  uint32_t small_max = 0;  // this thread's longest small segment
   for (uint32_t q = t; q < U; q += nt) {
     uint32_t s = a.act_slot[q];
     if (s == kNone32) continue;
@@ -1421,11 +1424,18 @@ __global__ void __launch_bounds__(1024) k_plan(PlanArgs<typename KeyTraits<T>::K
       const K klo = a.act_klo[q], khi = a.act_khi[q];
       a.small_start[j] = dst;
       a.small_len[j] = c;
+      small_max = c > small_max ? c : small_max;
       a.small_klo[j] = klo;
       a.small_khi[j] = khi;
       kfill[j] = 0;
     }
   }
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) {  // one LDS atomic per wave
+    const uint32_t o = (uint32_t)__shfl_xor((int)small_max, d, 64);
+    small_max = o > small_max ? o : small_max;
+  }
+  if (lane_id() == 0 && small_max) atomicMax(&sh_tot[7], small_max);
   __threadfence_block();
   __syncthreads();
 
@@ -1567,6 +1577,7 @@ __global__ void __launch_bounds__(1024) k_plan(PlanArgs<typename KeyTraits<T>::K
     cc->live_ks = live_tot;
     cc->U_wide = sh_tot[5];
     cc->U_wave = sh_tot[6];
+    cc->small_max = sh_tot[7];
     uint32_t nan = 0;
     if (ctl.nan_lo_bucket != kNone32) nan += a.totals[ctl.nan_lo_bucket];
     if (ctl.nan_hi_bucket != kNone32) nan += a.totals[ctl.nan_hi_bucket];

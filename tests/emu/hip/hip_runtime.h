@@ -79,6 +79,13 @@ template <class T> inline T atomicMin(T* p, T v) {
   }
   return cur;
 }
+template <class T> inline T atomicMax(T* p, T v) {
+  std::atomic_ref<T> a(*p);
+  T cur = a.load();
+  while (v > cur && !a.compare_exchange_weak(cur, v)) {
+  }
+  return cur;
+}
 inline uint32_t __umulhi(uint32_t a, uint32_t b) { return (uint32_t)(((uint64_t)a * b) >> 32); }
 inline uint64_t __ballot(int p) { return p ? 1ull : 0ull; }
 inline uint32_t __lane_id() { return threadIdx.x & 63; }
