@@ -594,14 +594,14 @@ __device__ __forceinline__ void stream_tile_vec(const V* __restrict__ x, uint32_
     if (v + 3 * nt < v1) {
       uint4 q[4];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) q[u] = xv[v + u * nt];
+      for (int u = 0; u < 4; ++u) q[u] = NT ? ld_stream(xv + v + u * nt) : xv[v + u * nt];
       for (;;) {
         const uint32_t vn = v + 4 * nt;
         const bool more = vn + 3 * nt < v1;
         uint4 r[4];
         if (more) {
 #pragma unroll
-          for (int u = 0; u < 4; ++u) r[u] = xv[vn + u * nt];
+          for (int u = 0; u < 4; ++u) r[u] = NT ? ld_stream(xv + vn + u * nt) : xv[vn + u * nt];
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
