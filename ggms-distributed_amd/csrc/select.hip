@@ -294,6 +294,9 @@ static size_t count_lds0_total(const Config& c) {
   return count_lds0<T>(c) + LutGeom<K>::bytes();
 }
 
+// at most this many ks: the setup runs inside the first-pass planning kernel
+constexpr uint32_t kSetupInPlan0MaxKs = kSetupKsPerThread * 1024;
+
 // LDS ceiling of one workgroup on gfx950 (160 KB per CU; static LDS of the
 // kernels stays under 2 KB).
 constexpr size_t kLdsLimit = 158 * 1024;
@@ -381,14 +384,17 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
   ggms_stats& stt = ctx->stats;
   memset(&stt, 0, sizeof(stt));
   Prof prof(ctx, st, o.profile != 0);
-  prof.begin(GGMS_KERNEL_OTHER, -1);
-  {
-    const uint32_t words = (uint32_t)((c.max_passes + 2) * ctlb / 4);
-    const uint32_t threads = std::max(words, nk);
-    k_setup<<<(threads + 255) / 256, 256, 0, st>>>(d_ks, nk, n, k_rank, k_state, ctl, words, c0, dbad);
+  // setup (control blocks, ks -> ranks): its own kernel, or folded into the
+  // first-pass planning kernel when the ks are few enough for one workgroup
+  const uint32_t ctl_words = (uint32_t)((c.max_passes + 2) * ctlb / 4);
+  const bool setup_in_plan0 = n > c.fin_cap && nk <= kSetupInPlan0MaxKs;
+  if (!setup_in_plan0) {
+    prof.begin(GGMS_KERNEL_OTHER, -1);
+    const uint32_t threads = std::max(ctl_words, nk);
+    k_setup<<<(threads + 255) / 256, 256, 0, st>>>(d_ks, nk, n, k_rank, k_state, ctl, ctl_words, c0, dbad);
+    prof.end();
+    HIP_TRY(hipGetLastError());
   }
-  prof.end();
-  HIP_TRY(hipGetLastError());
 
   const uint32_t fin_grid = std::max<uint32_t>(1, std::min<uint32_t>(4 * (uint32_t)ctx->cus, 65535));
   const size_t fin_lds = (size_t)(c.fin_cap / 2) * sizeof(K);  // two workgroups per CU
@@ -428,6 +434,14 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
     pa.strideS = 8192;
     pa.seed = c.seed;
     pa.mlp = c.mode == kModeLearned ? learned_weights_device(ctx->device) : nullptr;
+    pa.setup = setup_in_plan0 ? 1u : 0u;
+    pa.nk = nk;
+    pa.ctl_words = ctl_words;
+    pa.ks = d_ks;
+    pa.k_rank = k_rank;
+    pa.k_state = k_state;
+    pa.bad = dbad;
+    pa.c0 = c0;
     size_t lds_p0 = plan0_lds_bytes<K>(pow2ceil_host(c.S));
     LDS_CHECK(lds_p0);
     set_lds_attr((const void*)k_plan0<T>, lds_p0);

@@ -64,6 +64,7 @@ struct PassCtl {
   uint32_t pad[11];
 };
 
+constexpr uint32_t kSetupKsPerThread = 8;
 struct Plan0Args {
   uint32_t mode;        // SplitMode
   uint32_t S;           // sample size (<= 8192)
@@ -73,6 +74,16 @@ struct Plan0Args {
   uint32_t strideS;     // strided mode: first-level strided sample size (8192)
   uint64_t seed;
   const float* mlp;     // learned mode: MLP parameters (see mlp layout in select.hip)
+  // setup != 0: the kernel also does k_setup's work (nk <= kSetupKsPerThread
+  // * blockDim: saves a launch)
+  uint32_t setup;
+  uint32_t nk;
+  uint32_t ctl_words;
+  const uint32_t* ks;
+  uint32_t* k_rank;
+  uint32_t* k_state;
+  uint32_t* bad;
+  PassCtl c0;
 };
 
 // ---------------------------------------------------------------------------
@@ -1070,11 +1081,11 @@ static __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t words, uint
 // k-th smallest rank.  A k outside [1, n] sets *bad (a flag word in
 // host-visible memory) and is never selected.  ks may be host-visible memory.
 // ---------------------------------------------------------------------------
-static __global__ void k_setup(const uint32_t* __restrict__ ks, uint32_t nk, uint32_t n, uint32_t* __restrict__ k_rank,
-                               uint32_t* __restrict__ k_state, PassCtl* __restrict__ ctl, uint32_t ctl_words,
-                               PassCtl c0, uint32_t* __restrict__ bad) {
+__device__ __forceinline__ void setup_item(uint32_t i, const uint32_t* __restrict__ ks, uint32_t nk, uint32_t n,
+                                           uint32_t* __restrict__ k_rank, uint32_t* __restrict__ k_state,
+                                           PassCtl* __restrict__ ctl, uint32_t ctl_words, const PassCtl& c0,
+                                           uint32_t* __restrict__ bad) {
   constexpr uint32_t kCtlWords = sizeof(PassCtl) / 4;
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i == 0) ctl[0] = c0;
   else if (i >= kCtlWords && i < ctl_words) reinterpret_cast<uint32_t*>(ctl)[i] = 0u;
   if (i >= nk) return;
@@ -1087,6 +1098,12 @@ static __global__ void k_setup(const uint32_t* __restrict__ ks, uint32_t nk, uin
   }
   k_rank[i] = n - k + 1;
   k_state[i] = kLive;
+}
+
+static __global__ void k_setup(const uint32_t* __restrict__ ks, uint32_t nk, uint32_t n, uint32_t* __restrict__ k_rank,
+                               uint32_t* __restrict__ k_state, PassCtl* __restrict__ ctl, uint32_t ctl_words,
+                               PassCtl c0, uint32_t* __restrict__ bad) {
+  setup_item(blockIdx.x * blockDim.x + threadIdx.x, ks, nk, n, k_rank, k_state, ctl, ctl_words, c0, bad);
 }
 
 // ---------------------------------------------------------------------------

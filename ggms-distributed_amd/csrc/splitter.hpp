@@ -142,7 +142,23 @@ __global__ void __launch_bounds__(1024) k_plan0(const T* __restrict__ x, Plan0Ar
   using KT = KeyTraits<T>;
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ uint32_t bcast_R;
-  const uint32_t n = ctl->n;
+  // k_setup's work folded in (the host does it only for nk <= kSetupKsPerThread
+  // * blockDim): control blocks now, the ks' reads issued now and used after
+  // the sample gathers, so their round trip (host memory) overlaps those
+  constexpr uint32_t kCtlWords = sizeof(PassCtl) / 4;
+  uint32_t kv[kSetupKsPerThread];
+  if (a.setup) {
+    for (uint32_t i = threadIdx.x; i < a.ctl_words; i += blockDim.x) {
+      if (i == 0) ctl[0] = a.c0;
+      else if (i >= kCtlWords) reinterpret_cast<uint32_t*>(ctl)[i] = 0u;
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < kSetupKsPerThread; ++u) {
+      const uint32_t i = threadIdx.x + u * blockDim.x;
+      kv[u] = i < a.nk ? a.ks[i] : 0u;
+    }
+  }
+  const uint32_t n = a.setup ? a.c0.n : ctl->n;
   const uint32_t S = a.S;
   const uint32_t Spad = pow2ceil_u32(S);
   K* s = reinterpret_cast<K*>(smem);
@@ -197,6 +213,22 @@ __global__ void __launch_bounds__(1024) k_plan0(const T* __restrict__ x, Plan0Ar
         if (key > highest) key = highest;
       }
       s[i] = key;
+    }
+  }
+  if (a.setup) {
+#pragma unroll
+    for (uint32_t u = 0; u < kSetupKsPerThread; ++u) {
+      const uint32_t i = threadIdx.x + u * blockDim.x;
+      if (i >= a.nk) break;
+      const uint32_t k = kv[u];
+      if (k < 1 || k > n) {
+        *a.bad = 1u;
+        a.k_rank[i] = 1;
+        a.k_state[i] = kDone;
+      } else {
+        a.k_rank[i] = n - k + 1;
+        a.k_state[i] = kLive;
+      }
     }
   }
   __syncthreads();
