@@ -143,6 +143,24 @@ def test_errors(gpu_ctx):
     assert e.value.status == ggms.ENAN
 
 
+@pytest.mark.parametrize("nk", [8192, 8193, 20000])
+def test_setup_paths(gpu_ctx, nk):
+    """The setup step runs inside the first-pass planning kernel up to 8192 ks and as its own
+    kernel above: both answer exactly and both reject a k outside [1, n] (last position)."""
+    import ggms
+    n = 1 << 20
+    x = ggms.generate("float32", "normal", n, 77)
+    ks = ggms.generate_ks("uniform_random", nk, n, 78)
+    for alg in ("iterative", "bucket"):
+        _check(gpu_ctx, x, ks, alg, label=f"nk={nk}")
+    bad = ks.copy()
+    bad[-1] = n + 1
+    with pytest.raises(ggms.GGMSError) as e:
+        gpu_ctx.select(x, bad, "iterative")
+    assert e.value.status == ggms.EINVAL
+    _check(gpu_ctx, x, ks, "iterative", label=f"nk={nk} after EINVAL")
+
+
 def test_ulp_neighbours_regression(gpu_ctx):
     """Floats one ULP apart must be separated (DistributedSMOS/pivots and slopes:3-12 traces the
     reference failing to separate 1.101046334952116e-02 from 1.101046241819858e-02)."""
