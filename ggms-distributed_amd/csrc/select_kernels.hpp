@@ -1112,38 +1112,49 @@ static __global__ void k_setup(const uint32_t* __restrict__ ks, uint32_t nk, uin
 template <class T>
 __global__ void __launch_bounds__(1024) k_count0(const T* __restrict__ x, const Range<typename KeyTraits<T>::K>* __restrict__ ranges,
                                                  const typename KeyTraits<T>::K* __restrict__ lut,
-                                                 const PassCtl* __restrict__ ctl, uint32_t* __restrict__ rows) {
+                                                 const PassCtl* __restrict__ ctl, uint32_t* __restrict__ rows,
+                                                 uint32_t tiles_per_wg) {
   using K = typename KeyTraits<T>::K;
   extern __shared__ __align__(16) unsigned char smem[];
-  const uint32_t B = ctl->B, n = ctl->n, tile = ctl->tile;
-  if (blockIdx.x >= ctl->G) return;
+  const uint32_t B = ctl->B, n = ctl->n, tile = ctl->tile, G = ctl->G;
+  const uint32_t w0 = blockIdx.x * tiles_per_wg;
+  if (w0 >= G) return;
   // the cell table is a static LDS array: its address is a compile-time
   // constant folded into every cell read (a dynamic-LDS base costs an add per key)
   __shared__ __align__(16) K s_lut[LutGeom<K>::kEntries];
   Pass0Map<K> mp;
   mp.lut = s_lut;
   mp.set_params(smem, ctl->R);
-  uint32_t* hist = reinterpret_cast<uint32_t*>(smem + Pass0Map<K>::param_bytes(ctl->R));
-  for (uint32_t i = threadIdx.x; i < B; i += blockDim.x) hist[i] = 0;
+  // tiles_per_wg (1 or 2) consecutive tiles, each counted into its own
+  // histogram with no barrier between them, so the loads keep streaming
+  // across the tile boundary; one histogram row per tile
+  uint32_t* hist0 = reinterpret_cast<uint32_t*>(smem + Pass0Map<K>::param_bytes(ctl->R));
+  for (uint32_t i = threadIdx.x; i < B * tiles_per_wg; i += blockDim.x) hist0[i] = 0;
   mp.load(lut, ranges, ctl->R);
   __syncthreads();
-  uint32_t t0 = blockIdx.x * tile;
-  uint32_t t1 = t0 + tile < n ? t0 + tile : n;
-  constexpr int N = Vec16<T>::N;
-  stream_tile_vec_any<T, true>(
-      x, t0, t1,
-      [&](const T (&v)[N]) {
-        K k[N];
+  for (uint32_t h = 0; h < tiles_per_wg && w0 + h < G; ++h) {
+    uint32_t* hist = hist0 + h * B;
+    uint32_t t0 = (w0 + h) * tile;
+    uint32_t t1 = t0 + tile < n ? t0 + tile : n;
+    constexpr int N = Vec16<T>::N;
+    stream_tile_vec_any<T, true>(
+        x, t0, t1,
+        [&](const T (&v)[N]) {
+          K k[N];
 #pragma unroll
-        for (int j = 0; j < N; ++j) k[j] = KeyTraits<T>::enc(v[j]);
-        uint32_t b[N];
-        mp.template buckets<N>(k, b);
-        lds_count_n<N>(hist, b);
-      },
-      [&](T v) { lds_count(hist, mp.bucket(KeyTraits<T>::enc(v))); });
+          for (int j = 0; j < N; ++j) k[j] = KeyTraits<T>::enc(v[j]);
+          uint32_t b[N];
+          mp.template buckets<N>(k, b);
+          lds_count_n<N>(hist, b);
+        },
+        [&](T v) { lds_count(hist, mp.bucket(KeyTraits<T>::enc(v))); });
+  }
   __syncthreads();
-  uint32_t* row = rows + (size_t)blockIdx.x * B;
-  for (uint32_t i = threadIdx.x; i < B; i += blockDim.x) row[i] = hist[i];
+  for (uint32_t h = 0; h < tiles_per_wg && w0 + h < G; ++h) {
+    uint32_t* row = rows + (size_t)(w0 + h) * B;
+    const uint32_t* hist = hist0 + h * B;
+    for (uint32_t i = threadIdx.x; i < B; i += blockDim.x) row[i] = hist[i];
+  }
 }
 
 // ---------------------------------------------------------------------------
