@@ -167,6 +167,13 @@ __global__ void __launch_bounds__(1024) k_plan0(const T* __restrict__ x, Plan0Ar
   const K lowest = KT::kFloat ? KT::kNegInf : (K)0;
   const K highest = KT::kFloat ? KT::kPosInf : KeyMax<K>::v;
 
+  // the learned form's network weights go to LDS now, their reads overlapping
+  // the sample gathers (the area is not used before the pivots)
+  if (a.mode == kModeLearned) {
+    float* wl0 = reinterpret_cast<float*>(q + 2048 + kPlan0MaxRanges);
+    for (uint32_t i = threadIdx.x; i < (uint32_t)kMlpParams; i += blockDim.x) wl0[i] = a.mlp[i];
+  }
+
   // 1. sample: up to 8 positions per thread, all loads issued before the
   //    first use (random reads over the whole vector: each one is a full
   //    memory round trip, so they must overlap)
@@ -262,13 +269,9 @@ __global__ void __launch_bounds__(1024) k_plan0(const T* __restrict__ x, Plan0Ar
   } else {
     const uint32_t Pl = a.P;  // pieces
     Q = Pl + 1;
-    // network weights staged in LDS (every thread reads the same weight at the
-    // same time: broadcast reads)
+    // network weights, staged in LDS at the start (every thread reads the same
+    // weight at the same time: broadcast reads)
     float* wl = reinterpret_cast<float*>(q + 2048 + kPlan0MaxRanges);
-    if (a.mode == kModeLearned) {
-      for (uint32_t i = threadIdx.x; i < (uint32_t)kMlpParams; i += blockDim.x) wl[i] = a.mlp[i];
-      __syncthreads();
-    }
     if (a.mode == kModeLearned) {
       learned_pivots<K>(s, S, Pl, wl, q, reinterpret_cast<float*>(wl + kMlpParams));
     } else {
