@@ -458,12 +458,20 @@ __global__ void __launch_bounds__(1024) k_batch(BatchArgs a) {
             for (int j = 0; j < NV; ++j) k[j] = KT::enc(v[j]);
             uint32_t b[NV];
             mp.template buckets<NV>(k, b);
+            uint32_t s[NV];
+            bool any = false;
 #pragma unroll
             for (int j = 0; j < NV; ++j) {
-              uint32_t s = slot[b[j]];
-              bool act = s != kNone16;
-              uint32_t pos = lds_claim(L.cursor, s, act);
-              if (act) sA[pos] = k[j];
+              s[j] = slot[b[j]];
+              any |= s[j] != kNone16;
+            }
+            // survivors are a few percent of the keys and rarely share a child
+            // within a wave: one LDS atomic per survivor, nothing for a wave
+            // without any
+            if (wave_ballot(any)) {
+#pragma unroll
+              for (int j = 0; j < NV; ++j)
+                if (s[j] != kNone16) sA[atomicAdd(&L.cursor[s[j]], 1u)] = k[j];
             }
           },
           [&](T v) {
@@ -614,8 +622,18 @@ __global__ void __launch_bounds__(1024) k_batch(BatchArgs a) {
           stream_tile_vec_any<K, false, true>(
               src, 0, it.len,
               [&](const K (&k)[NK]) {
+                uint32_t s[NK];
+                bool any = false;
 #pragma unroll
-                for (int j = 0; j < NK; ++j) claim(k[j]);
+                for (int j = 0; j < NK; ++j) {
+                  s[j] = slot1[map_off<K>(k[j], rr.lo, rr.sh, rr.mult)];
+                  any |= s[j] != kNone16;
+                }
+                if (wave_ballot(any)) {  // as in the first pass: one LDS atomic per survivor
+#pragma unroll
+                  for (int j = 0; j < NK; ++j)
+                    if (s[j] != kNone16) dst[atomicAdd(&L.cursor[s[j]], 1u)] = k[j];
+                }
               },
               claim);
           __threadfence_block();
