@@ -1856,13 +1856,18 @@ __global__ void __launch_bounds__(1024) k_compact1(const K* __restrict__ buf, co
         uint32_t b[N];
         pm.template buckets<N>(k, b);
         uint32_t sl[N];
-#pragma unroll
-        for (int j = 0; j < N; ++j) sl[j] = ct.slot[b[j]];
+        bool any = false;
 #pragma unroll
         for (int j = 0; j < N; ++j) {
-          bool act = sl[j] != kNone16;
-          uint32_t p = lds_claim(ct.cursor, sl[j], act);
-          if (act) out[p] = k[j];
+          sl[j] = ct.slot[b[j]];
+          any |= sl[j] != kNone16;
+        }
+        // one LDS atomic per survivor (no duplicate peeling: the survivors of
+        // a wave rarely share a segment), nothing for a wave without any
+        if (wave_ballot(any)) {
+#pragma unroll
+          for (int j = 0; j < N; ++j)
+            if (sl[j] != kNone16) out[atomicAdd(&ct.cursor[sl[j]], 1u)] = k[j];
         }
       },
       [&](K key) {
