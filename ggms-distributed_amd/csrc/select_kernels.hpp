@@ -1305,10 +1305,19 @@ __global__ void __launch_bounds__(1024) k_plan(PlanArgs<typename KeyTraits<T>::K
       rk_bits[w] = 0;
       rk_first[w] = kNone32;
     }
-  for (uint32_t i = t; i < B; i += nt) {
-    P[i] = a.totals[i];
-    aidx[i] = 0;
-    a.slot_of_bucket[i] = (uint16_t)kNone16;
+  for (uint32_t i0 = t; i0 < B; i0 += 8 * nt) {  // eight reads in flight per thread
+    uint32_t v[8];
+#pragma unroll
+    for (uint32_t u = 0; u < 8; ++u) v[u] = i0 + u * nt < B ? a.totals[i0 + u * nt] : 0u;
+#pragma unroll
+    for (uint32_t u = 0; u < 8; ++u) {
+      const uint32_t i = i0 + u * nt;
+      if (i < B) {
+        P[i] = v[u];
+        aidx[i] = 0;
+        a.slot_of_bucket[i] = (uint16_t)kNone16;
+      }
+    }
   }
   __syncthreads();
   block_incl_scan_lds(P, B, scratch);
