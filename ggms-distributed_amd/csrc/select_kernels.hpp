@@ -488,12 +488,22 @@ template <class K>
 __host__ __device__ __forceinline__ bool finish_wide_segment(uint32_t len, K klo, K khi, uint32_t fin_cap) {
   return len <= fin_cap && !finish_fits_half<K>(len, klo, khi, fin_cap);
 }
+// The part of a segment's class that its length and key interval decide:
+// low 8 bits the class without the one-wave option, bit 8 set when its
+// length allows the one-wave class; order_bin adds the ks' count.
+template <class K>
+__host__ __device__ __forceinline__ uint32_t order_bin_info(uint32_t len, K klo, K khi, uint32_t fin_cap) {
+  if (finish_wide_segment<K>(len, klo, khi, fin_cap)) return 0;
+  const uint32_t base = len >> 10 < kOrderBins - 3 ? 2 + (len >> 10) : kOrderBins - 1;
+  return base | (len <= kFinishWaveLen ? 0x100u : 0u);
+}
+__host__ __device__ __forceinline__ uint32_t order_bin_of(uint32_t info, uint32_t nks, bool waves) {
+  return (waves && (info & 0x100u) && nks <= kFinishWaveKs) ? 1u : (info & 0xFFu);
+}
 template <class K>
 __host__ __device__ __forceinline__ uint32_t order_bin(uint32_t len, K klo, K khi, uint32_t fin_cap, uint32_t nks,
                                                        bool waves) {
-  if (finish_wide_segment<K>(len, klo, khi, fin_cap)) return 0;
-  if (waves && len <= kFinishWaveLen && nks <= kFinishWaveKs) return 1;
-  return len >> 10 < kOrderBins - 3 ? 2 + (len >> 10) : kOrderBins - 1;
+  return order_bin_of(order_bin_info<K>(len, klo, khi, fin_cap), nks, waves);
 }
 
 // words of a slot rank table (32 buckets per 64-bit word)
@@ -1426,7 +1436,13 @@ __global__ void __launch_bounds__(1024) k_plan(PlanArgs<typename KeyTraits<T>::K
   if (Bn < 4 * U_big) Bn = 4 * U_big;
   if (Bn > a.B_cap) Bn = a.B_cap;
 
-  // P5: slot tables, small segment descriptors, next ranges (unplaced)
+  // P5: slot tables, small segment descriptors, next ranges (unplaced).
+  // With room in LDS (behind kfill in P, and aidx once P7 is done) the small
+  // segments' class information and k-list starts are kept there for P8, so
+  // it does not read back what this workgroup just wrote to global memory.
+  const bool lcache = 2 * U_small <= B && U_small < B;
+  uint32_t* sinfo = P + U_small;  // class information per small segment
+  uint32_t* kst = aidx;           // k-list start per small segment (from P8 on)
   uint32_t small_max = 0;  // this thread's longest small segment
   for (uint32_t q = t; q < U; q += nt) {
     uint32_t s = a.act_slot[q];
@@ -1464,6 +1480,7 @@ __global__ void __launch_bounds__(1024) k_plan(PlanArgs<typename KeyTraits<T>::K
       small_max = c > small_max ? c : small_max;
       a.small_klo[j] = klo;
       a.small_khi[j] = khi;
+      if (lcache) sinfo[j] = order_bin_info<K>(c, klo, khi, a.fin_cap);
       kfill[j] = 0;
     }
   }
@@ -1529,22 +1546,31 @@ __global__ void __launch_bounds__(1024) k_plan(PlanArgs<typename KeyTraits<T>::K
   __syncthreads();
 
   // P8: k lists of the small segments (counting sort by segment)
+  const bool waves = U_small >= kFinishWaveMinSegs;
+  auto binfo = [&](uint32_t j) {
+    return lcache ? sinfo[j] : order_bin_info<K>(a.small_len[j], a.small_klo[j], a.small_khi[j], a.fin_cap);
+  };
+  auto kstart = [&](uint32_t j) { return lcache ? kst[j] : a.small_kstart[j]; };
   {
     uint32_t per = (U_small + nt - 1) / nt;
     uint32_t j0 = t * per, j1 = j0 + per < U_small ? j0 + per : U_small;
     uint32_t s = 0;
     for (uint32_t j = j0; j < j1; ++j) {
       s += kfill[j];
-      atomicAdd(&ord_cnt[order_bin<K>(a.small_len[j], a.small_klo[j], a.small_khi[j], a.fin_cap, kfill[j], U_small >= kFinishWaveMinSegs)], 1u);
+      atomicAdd(&ord_cnt[order_bin_of(binfo(j), kfill[j], waves)], 1u);
     }
     uint32_t tot;
     uint32_t off = block_excl_scan_u32(s, scratch, &tot);
     for (uint32_t j = j0; j < j1; ++j) {
       a.small_kstart[j] = off;
+      if (lcache) kst[j] = off;
       off += kfill[j];
       kfill[j] = 0;
     }
-    if (t == 0) a.small_kstart[U_small] = tot;
+    if (t == 0) {
+      a.small_kstart[U_small] = tot;
+      if (lcache) kst[U_small] = tot;
+    }
   }
   __threadfence_block();
   __syncthreads();
@@ -1564,7 +1590,7 @@ __global__ void __launch_bounds__(1024) k_plan(PlanArgs<typename KeyTraits<T>::K
     if (st == kSmallNew) {
       uint32_t j = a.k_aux[k];
       uint32_t pos = atomicAdd(&kfill[j], 1u);
-      a.klist[a.small_kstart[j] + pos] = k;
+      a.klist[kstart(j) + pos] = k;
       a.k_state[k] = kSmall;
     }
     if (st == kLive) live++;
@@ -1574,8 +1600,8 @@ __global__ void __launch_bounds__(1024) k_plan(PlanArgs<typename KeyTraits<T>::K
   // finish order of the small segments: larger ones first, so the longest
   // workgroups start early instead of trailing the launch
   for (uint32_t j = t; j < U_small; j += nt) {
-    const uint32_t nks = a.small_kstart[j + 1] - a.small_kstart[j];
-    a.small_order[atomicAdd(&ord_cnt[order_bin<K>(a.small_len[j], a.small_klo[j], a.small_khi[j], a.fin_cap, nks, U_small >= kFinishWaveMinSegs)], 1u)] = j;
+    const uint32_t nks = kstart(j + 1) - kstart(j);
+    a.small_order[atomicAdd(&ord_cnt[order_bin_of(binfo(j), nks, waves)], 1u)] = j;
   }
 
   // P9: next-pass tiling and first range of every tile
