@@ -48,35 +48,42 @@ def bench_other_configs(ctx, args):
     2^26 adversarial inputs with 8192 evenly spaced or clustered ranks, a 2^28 fp32 Gaussian
     mixture with 8192 random ranks, and 2^27 fp64 N(0,1) with 1001 random ranks.  Exactness at
     these sizes is checked by tests/test_gpu_configs.py."""
+    import numpy as np
     import torch
     import ggms
-    cases = [("uint32 U[0,1023]", "uint32", "uniform", 26, 1024.0, 0.0, "even", 8192),
-             ("uint32 U[0,1023]", "uint32", "uniform", 26, 1024.0, 0.0, "window", 8192),
-             ("fp32 |N(0,1)|", "float32", "half_normal", 26, 0.0, 0.0, "even", 8192),
-             ("fp32 99% 1.0f", "float32", "near_constant", 26, 1.0, 0.01, "even", 8192),
-             ("fp32 99% 1.0f", "float32", "near_constant", 26, 1.0, 0.01, "window", 8192),
-             ("fp32 mixture", "float32", "mixture", 28, 0.0, 0.0, "uniform_random", 8192),
-             ("fp64 N(0,1)", "float64", "normal", 27, 0.0, 0.0, "uniform_random", 1001)]
+    lin = ("linear",)
+    cases = [("uint32 U[0,1023]", "uint32", "uniform", 26, 1024.0, 0.0, "even", 8192, lin),
+             ("uint32 U[0,1023]", "uint32", "uniform", 26, 1024.0, 0.0, "window", 8192, lin),
+             ("fp32 |N(0,1)|", "float32", "half_normal", 26, 0.0, 0.0, "even", 8192, lin),
+             ("fp32 99% 1.0f", "float32", "near_constant", 26, 1.0, 0.01, "even", 8192, lin),
+             ("fp32 99% 1.0f", "float32", "near_constant", 26, 1.0, 0.01, "window", 8192, lin),
+             ("fp32 mixture", "float32", "mixture", 28, 0.0, 0.0, "uniform_random", 8192,
+              ("linear", "sketch", "learned")),
+             ("fp64 N(0,1)", "float64", "normal", 27, 0.0, 0.0, "uniform_random", 1001, lin)]
     out = []
-    for label, dt, dist, lg, p0, p1, kd, nk in cases:
+    for label, dt, dist, lg, p0, p1, kd, nk, splitters in cases:
         n = 1 << lg
         x = ggms.generate(dt, dist, n, args.seed + lg, p0, p1)
         ks = ggms.generate_ks(kd, nk, n, args.seed + nk)
-        o = ggms.default_opts(splitter="linear")
-        ctx.select(x, ks, "iterative", o)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        reps = 5
-        for _ in range(reps):
-            ctx.select(x, ks, "iterative", o)
-        torch.cuda.synchronize()
-        ms = (time.perf_counter() - t0) / reps * 1e3
-        ctx.select(x, ks, "iterative", ggms.default_opts(splitter="linear", profile=1))
-        st = ctx.stats()
-        out.append({"workload": f"{label}, N=2^{lg}, {nk} ks {kd}", "ms_per_step": round(ms, 4),
-                    "elements_per_s": round(n / (ms * 1e-3), 1), "passes": st["passes"],
-                    "n_kept": st["n_kept"], "buckets": st["buckets"],
-                    "kernel_ms": {k: round(v, 4) for k, v in st["kernel_ms"].items() if v > 0}})
+        ref = None
+        for sp in splitters:
+            o = ggms.default_opts(splitter=sp)
+            got = ctx.select(x, ks, "iterative", o)
+            ref = got if ref is None else ref
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            reps = 5
+            for _ in range(reps):
+                ctx.select(x, ks, "iterative", o)
+            torch.cuda.synchronize()
+            ms = (time.perf_counter() - t0) / reps * 1e3
+            ctx.select(x, ks, "iterative", ggms.default_opts(splitter=sp, profile=1))
+            st = ctx.stats()
+            out.append({"workload": f"{label}, N=2^{lg}, {nk} ks {kd}", "splitter": sp, "ms_per_step": round(ms, 4),
+                        "elements_per_s": round(n / (ms * 1e-3), 1), "passes": st["passes"],
+                        "n_kept": st["n_kept"], "buckets": st["buckets"],
+                        "identical_to_linear": bool((got.view(np.uint8) == ref.view(np.uint8)).all()),
+                        "kernel_ms": {k: round(v, 4) for k, v in st["kernel_ms"].items() if v > 0}})
         del x
     return out
 
@@ -142,6 +149,9 @@ def main():
                          "hardware byte counters can be scaled")
     ap.add_argument("--no-forms", action="store_true", help="skip the learned and sketch forms")
     ap.add_argument("--no-extra", action="store_true", help="skip the adversarial / mixture / fp64 workloads")
+    ap.add_argument("--ref-csv", metavar="DIR", default=None,
+                    help="also write reference-format result rows (one per step, every algorithm beside a GPU "
+                         "sort-and-choose) and their Summary / Overview / Ratio files into DIR")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: the multi-GPU form (one histogram all-reduce "
@@ -183,30 +193,51 @@ def main():
     ms_per_step = 1000.0 * tall_lin / args.steps
     value = n / (tall_lin / args.steps)
 
-    # profiled call: per-kernel durations from HIP events on the launch stream
-    ctx.select(x, ks, "iterative", ggms.default_opts(splitter="linear", profile=1))
-    st = ctx.stats()
+    # profiled calls: per-kernel durations from HIP events recorded around every
+    # launch on the launch stream, averaged over PROFILED calls
+    PROFILED = 5
+    sums = {}
+    for _ in range(PROFILED):
+        ctx.select(x, ks, "iterative", ggms.default_opts(splitter="linear", profile=1))
+        st = ctx.stats()
+        for p in range(st["passes"]):
+            for kind, key in (("count", "count_ms"), ("compact", "compact_ms")):
+                sums[(kind, p)] = sums.get((kind, p), 0.0) + st[key][p]
+        for k, v in st["kernel_ms"].items():
+            sums[k] = sums.get(k, 0.0) + v
     bpe = 4
     passes = []
+    pass_kernels = []  # (name, algorithmic bytes per launch, average ms)
     for p in range(st["passes"]):
         n_in = st["n_in"][p]
         kept = st["n_kept"][p]
-        c_ms = st["count_ms"][p]
-        k_ms = st["compact_ms"][p]
+        c_ms = sums[("count", p)] / PROFILED
+        k_ms = sums[("compact", p)] / PROFILED
+        c_name = "k_count0<float>" if p == 0 else ("k_count1<unsigned int>" if p == 1 else "k_countS<unsigned int>")
+        k_name = "k_compact0<float>" if p == 0 else ("k_compact1<unsigned int>" if p == 1 else "k_compactS<unsigned int>")
+        pass_kernels.append((c_name, n_in * bpe, c_ms, p))
+        pass_kernels.append((k_name, (n_in + kept) * bpe, k_ms, p))
         passes.append({
             "pass": p, "n_in": n_in, "n_kept": kept, "active": st["active"][p], "buckets": st["buckets"][p],
             "count_ms": round(c_ms, 4), "compact_ms": round(k_ms, 4),
             "count_GBs": round(n_in * bpe / (c_ms * 1e6), 1) if c_ms > 0 else None,
             "compact_GBs": round((n_in + kept) * bpe / (k_ms * 1e6), 1) if k_ms > 0 else None,
         })
-    # dominant kernel: first-pass count (algorithmic bytes = N * 4 per launch)
-    c0_ms = st["count_ms"][0]
-    achieved = n * bpe / (c0_ms * 1e6)
-    traffic, traffic_src = measured_traffic("k_count0<float>") if args.log2n == 28 else (None, None)
-    roofline = {"bound": "hbm", "kernel": "k_count0<float>", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+    # dominant kernel: the longest bucketing-pass launch; algorithmic bytes per
+    # launch: count n_p * 4, compaction (n_p + n_{p+1}) * 4
+    dom = max(pass_kernels, key=lambda t: t[2])
+    d_name, d_bytes, d_ms, _ = dom
+    achieved = d_bytes / (d_ms * 1e6)
+    traffic, traffic_src = measured_traffic(d_name) if args.log2n == 28 else (None, None)
+    roofline = {"bound": "hbm", "kernel": d_name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "traffic_source": traffic_src,
-                "algorithmic_bytes_per_launch": n * bpe, "avg_launch_ms": round(c0_ms, 4)}
+                "algorithmic_bytes_per_launch": d_bytes, "avg_launch_ms": round(d_ms, 4),
+                "timing": f"HIP events around each launch on its stream, mean of {PROFILED} calls",
+                "pass_kernels": [{"kernel": nm, "pass": p, "algorithmic_bytes": b, "avg_ms": round(ms, 4),
+                                  "GBs": round(b / (ms * 1e6), 1) if ms > 0 else None,
+                                  "frac": round(b / (ms * 1e6) / HBM_PEAK_GBS, 4) if ms > 0 else None}
+                                 for nm, b, ms, p in pass_kernels]}
     # plain streaming references on the same buffer: a torch sum (reads 4N bytes) and a device copy
     # (reads and writes 4N bytes), timed with events
     probe = {}
@@ -244,7 +275,7 @@ def main():
                    "n": n, "nk": args.nk, "seed": args.seed, "alg": "iterative", "splitter": "linear"},
         "roofline": roofline,
         "passes": passes,
-        "kernel_ms": {k: round(v, 4) for k, v in st["kernel_ms"].items() if v > 0},
+        "kernel_ms": {k: round(sums[k] / PROFILED, 4) for k in st["kernel_ms"] if sums[k] > 0},
         "step_ms_median": round(1000 * statistics.median(t_lin), 4),
     }
 
@@ -264,6 +295,50 @@ def main():
                             "kernel_ms": {k: round(v, 4) for k, v in sf["kernel_ms"].items() if v > 0}}
         except ggms.GGMSError as e:
             result[form] = {"value": None, "note": str(e)}
+
+    # the device-driven schedule (every pass enqueued at once, no host read
+    # between kernels): as a blocking call, and captured once in a hipGraph
+    # (ggms_select_device) and replayed
+    if not args.no_forms:
+        dev_opts = ggms.default_opts(splitter="linear", schedule=1)
+        out_d, t_d, tall_d = timed(dev_opts, args.steps, args.warmup)
+        entry = {"value": round(n / (tall_d / args.steps), 1), "ms_per_step": round(1000.0 * tall_d / args.steps, 4),
+                 "speedup_vs_host_schedule": round(tall_lin / tall_d, 4),
+                 "identical_outputs": bool(np.array_equal(out_d.view(np.uint32), out_lin.view(np.uint32)))}
+        try:
+            import ctypes as C
+            lib = ggms._lib.load()
+            d_ks = torch.from_numpy(ks.view(np.int32)).cuda()
+            d_out = torch.empty(len(ks), dtype=torch.float32, device="cuda")
+            assert lib.ggms_ctx_reserve(ctx.h, 0, n, len(ks), C.byref(dev_opts)) == 0
+            ctx.select_device(x, d_ks, d_out, "iterative", dev_opts)
+            torch.cuda.synchronize()
+            gs = torch.cuda.Stream()
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g, stream=gs, capture_error_mode="thread_local"):
+                oc = ggms.default_opts(splitter="linear", schedule=1, stream=torch.cuda.current_stream().cuda_stream)
+                rc = lib.ggms_select_device(ctx.h, 1, 0, C.c_void_p(x.data_ptr()), n, C.c_void_p(d_ks.data_ptr()),
+                                            len(ks), C.c_void_p(d_out.data_ptr()), C.byref(oc))
+            assert rc == 0
+            for _ in range(args.warmup):
+                g.replay()
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(args.steps):
+                g.replay()
+            e1.record()
+            torch.cuda.synchronize()
+            gms = e0.elapsed_time(e1) / args.steps
+            entry["graph_replay"] = {"ms_per_step": round(gms, 4), "value": round(n / (gms * 1e-3), 1),
+                                     "status_ok": lib.ggms_ctx_device_status(ctx.h) == 0,
+                                     "identical_outputs": bool(np.array_equal(d_out.cpu().numpy().view(np.uint32),
+                                                                              out_lin.view(np.uint32))),
+                                     "note": "outputs stay in device memory (no host copy)"}
+            del g
+        except Exception as e:  # reported, never fatal for the headline
+            entry["graph_replay"] = {"error": repr(e)}
+        result["device_schedule"] = entry
 
     # the same linear splitter at the reference's bucket count
     if not args.no_forms:
@@ -288,30 +363,40 @@ def main():
         result["exact_vs_sort_and_choose"] = bool(np.array_equal(want.view(np.uint32), out_lin.view(np.uint32)))
 
     if not args.no_cpu:
+        # CPU sort-and-choose on the same input and ks: order-preserving keys,
+        # __gnu_parallel::sort, pick sorted[n - k]; median of 5 after 1 warm-up
         import oracle
         threads = min(16, os.cpu_count() or 1)
-        m = min(n, 1 << 26)
-        xs = x[:m].cpu().numpy()
-        ks_s = ggms.generate_ks("uniform_random", args.nk, m, args.seed + 1)
-        oracle.sort_and_choose(xs[: 1 << 20], ks_s[ks_s <= (1 << 20)], threads=threads)  # warm-up
+        xs = x.cpu().numpy()
+        oracle.sort_and_choose(xs, ks, threads=threads)  # warm-up
         tt = []
-        for _ in range(3):
+        for _ in range(5):
             t0 = time.perf_counter()
-            oracle.sort_and_choose(xs, ks_s, threads=threads)
+            oracle.sort_and_choose(xs, ks, threads=threads)
             tt.append(time.perf_counter() - t0)
         t_nth = []
         for _ in range(3):
             t0 = time.perf_counter()
-            oracle.nth_multiselect(xs, ks_s, threads=threads)
+            oracle.nth_multiselect(xs, ks, threads=threads)
             t_nth.append(time.perf_counter() - t0)
         result["cpu_baseline"] = {
-            "value": round(m / statistics.median(tt), 1), "unit": "elements/s", "cores": threads,
+            "value": round(n / statistics.median(tt), 1), "unit": "elements/s", "cores": threads,
             "kind": "port",
-            "sample": f"first 2^{m.bit_length() - 1} elements of the same input, {args.nk} uniform random ks; "
-                      f"order-preserving keys + __gnu_parallel::sort + pick; median of 3",
-            "nth_element_multiselect_value": round(m / statistics.median(t_nth), 1),
+            "sample": f"the same 2^{args.log2n} input and {args.nk} ks; order-preserving keys + "
+                      f"__gnu_parallel::sort + pick; median of 5 after 1 warm-up",
+            "ms_per_call": round(1000 * statistics.median(tt), 1),
+            "nth_element_multiselect_value": round(n / statistics.median(t_nth), 1),
             "host_cpus": os.cpu_count(),
         }
+        del xs
+    if args.ref_csv:
+        from ggms import refbench
+        path = refbench.sweep("float32", args.dist, "uniform_random", args.log2n, args.log2n, args.nk, 1, args.nk,
+                              args.steps, ("bucket", "iterative", "distributed", "distributed_bucket"),
+                              args.ref_csv, seed=args.seed, log=lambda s: print(s, file=sys.stderr, flush=True))
+        base = os.path.basename(path)
+        result["ref_csv"] = {"file": base, "summaries": [os.path.basename(q) for q in
+                                                         refbench.summarize(base, args.ref_csv)]}
     print(json.dumps(result), flush=True)
 
 

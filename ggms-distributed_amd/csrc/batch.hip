@@ -716,7 +716,7 @@ ggms_status ggms_batch_plan_create(int device, ggms_dtype t, const void* d_value
   if ((unsigned)t > 3u) return GGMS_EINVAL;
   *out_plan = nullptr;
   for (uint32_t p = 0; p < num_problems; ++p) {
-    if (h_offsets[p + 1] < h_offsets[p] || h_offsets[p + 1] - h_offsets[p] > 0xFFFFFFFFull) return GGMS_EINVAL;
+    if (h_offsets[p + 1] < h_offsets[p] || h_offsets[p + 1] - h_offsets[p] > kMaxN) return GGMS_EINVAL;
     if (h_koffsets[p + 1] < h_koffsets[p] || h_koffsets[p + 1] - h_koffsets[p] > kBatchMaxKs) return GGMS_EINVAL;
   }
   const uint64_t total = h_offsets[num_problems];

@@ -97,6 +97,10 @@ template <class K> struct Range {
 
 GGMS_HD uint32_t bitlen64(uint64_t w) { return w ? 64u - (uint32_t)__builtin_clzll(w) : 0u; }
 
+// End of the tile starting at t0 (t0 < n): min(t0 + tile, n) without forming
+// t0 + tile, which could pass 2^32 on the last tile.
+GGMS_HD uint32_t tile_end(uint32_t t0, uint32_t tile, uint32_t n) { return n - t0 > tile ? t0 + tile : n; }
+
 // Fill nb/sh/mult of r for the requested bucket count (>= 1).
 template <class K> GGMS_HD void make_map(Range<K>& r, uint32_t nb_req) {
   uint64_t w = (uint64_t)(r.hi - r.lo);

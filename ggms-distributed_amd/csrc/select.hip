@@ -65,6 +65,8 @@ struct Config {
   uint32_t fin_cap;
   uint64_t seed;
   uint32_t force_finish;
+  bool one_pass;       // bucketMultiselect: one bucketing pass, every k-bucket finished
+  bool device_schedule;  // every launch enqueued up front, no host reads between kernels
 };
 
 static uint32_t pow2ceil_host(uint32_t v) {
@@ -77,7 +79,8 @@ template <class K>
 static Config resolve_config(ggms_alg alg, uint64_t n, uint32_t nk, const ggms_opts& o, int cus) {
   Config c{};
   c.mode = kModeLinear;
-  if (alg == GGMS_ALG_DISTRIBUTED_1GPU) c.mode = kModeStrided;
+  if (alg == GGMS_ALG_DISTRIBUTED_1GPU || alg == GGMS_ALG_DISTRIBUTED_BUCKET_1GPU) c.mode = kModeStrided;
+  c.one_pass = alg == GGMS_ALG_BUCKET || alg == GGMS_ALG_DISTRIBUTED_BUCKET_1GPU;
   if (o.splitter == GGMS_SPLIT_LEARNED) c.mode = kModeLearned;
   if (o.splitter == GGMS_SPLIT_SKETCH) c.mode = kModeSketch;
   // 0 = automatic: 14336 for every splitter (the reference uses 8192).  More
@@ -112,8 +115,12 @@ static Config resolve_config(ggms_alg alg, uint64_t n, uint32_t nk, const ggms_o
   // passes >= 1: one workgroup per CU (their LDS tables and scattered writes
   // favour it: pass-1 count 0.059 -> 0.052 ms, compaction 0.085 -> 0.076 ms)
   c.G_max_next = o.blocks_per_cu ? c.G_max : std::max<uint32_t>(1, std::min<uint32_t>((uint32_t)cus, kMaxWorkgroups));
-  c.min_tile = 16384;
+  c.min_tile = kMinTile;
+  c.device_schedule = o.schedule == GGMS_SCHEDULE_DEVICE;
   c.max_passes = o.max_passes ? std::min<uint32_t>(o.max_passes, GGMS_MAX_PASS_STATS) : 64;
+  // the device-driven schedule launches every pass it may need: 3 by default
+  // (2^28 fp32 with 1001 ks needs 2, 8192 ks 3); the last finishes whatever is left
+  if (c.device_schedule) c.max_passes = o.device_passes ? std::min<uint32_t>(o.device_passes, 8) : 3;
   if (c.max_passes < 2) c.max_passes = 2;  // the first pass always hands its k-buckets to a second
   c.fin_cap = sizeof(K) == 4 ? 32768 : 16384;  // 128 KB of LDS in the finishing kernel
   c.seed = o.seed;
@@ -121,6 +128,16 @@ static Config resolve_config(ggms_alg alg, uint64_t n, uint32_t nk, const ggms_o
   (void)n;
   (void)nk;
   return c;
+}
+
+// Tiling of the first pass: at most G_max workgroups, each owning a tile of
+// at least min_tile elements (a multiple of 64), the last tile ragged.
+static void first_pass_tiling(uint64_t n, uint32_t G_max, uint32_t min_tile, uint32_t* tile, uint32_t* G) {
+  uint64_t g = std::min<uint64_t>(G_max, (n + min_tile - 1) / min_tile);
+  if (g < 1) g = 1;
+  const uint64_t t = ((n + g - 1) / g + 63) & ~63ull;
+  *tile = (uint32_t)t;
+  *G = (uint32_t)((n + t - 1) / t);
 }
 
 struct Workspace {
@@ -240,8 +257,11 @@ static Layout make_layout(uint64_t n, uint32_t nk, const Config& c) {
   size_t nkp = std::max<uint32_t>(nk, 1);
   size_t nbuf = n <= c.fin_cap ? n : n;
   sz[bBufA] = nbuf * sizeof(K);
-  sz[bBufB] = n <= c.fin_cap ? 0 : nbuf * sizeof(K);
-  sz[bRows] = (size_t)c.G_max * Bm * 4;
+  sz[bBufB] = nbuf * sizeof(K);  // also the spare buffer of the finish's global-memory sort
+  // one histogram row per workgroup: the first pass's tiles, or one per CU after it
+  uint32_t tile0 = 0, G0 = 1;
+  if (n > 0 && n <= kMaxN) first_pass_tiling(n, c.G_max, c.min_tile, &tile0, &G0);
+  sz[bRows] = (size_t)std::max(G0, c.G_max_next) * Bm * 4;
   sz[bTotals] = Bm * 4;
   sz[bRanges0] = Rm * sizeof(Range<K>);
   sz[bRanges1] = Rm * sizeof(Range<K>);
@@ -303,8 +323,25 @@ constexpr size_t kLdsLimit = 158 * 1024;
 // dynamic LDS that leaves room for only one 1024-thread workgroup per CU
 constexpr size_t kLdsOneWgPerCu = 81 * 1024;
 
+// Raises a kernel's dynamic-LDS limit on the current device when a launch
+// needs more than the default 64 KB; each (device, kernel) pair is raised once
+// to what it has needed so far, so repeated calls (and graph capture) make no
+// runtime call.
 static void set_lds_attr(const void* fn, size_t bytes) {
-  if (bytes > 65536) (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  if (bytes <= 65536) return;
+  static std::mutex mu;
+  static std::vector<std::pair<std::pair<int, const void*>, size_t>> done;
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  std::lock_guard<std::mutex> lk(mu);
+  for (auto& e : done)
+    if (e.first.first == dev && e.first.second == fn) {
+      if (e.second >= bytes) return;
+      if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) == hipSuccess) e.second = bytes;
+      return;
+    }
+  if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) == hipSuccess)
+    done.push_back({{dev, fn}, bytes});
 }
 
 #define LDS_CHECK(bytes)                              \
@@ -312,11 +349,286 @@ static void set_lds_attr(const void* fn, size_t bytes) {
     if ((bytes) > kLdsLimit) return GGMS_ENOTSUP;     \
   } while (0)
 
+// Arguments of the first-pass planning kernel (sampling, pivots, range table;
+// optionally the setup step).
+template <class T>
+static Plan0Args plan0_args(const Config& c, int device, uint32_t nk, uint32_t ctl_words, bool setup_in_plan0,
+                            const uint32_t* d_ks, uint32_t* k_rank, uint32_t* k_state, uint32_t* dbad,
+                            const PassCtl& c0) {
+  Plan0Args pa{};
+  pa.mode = c.mode;
+  pa.S = c.S;
+  pa.P = c.P;
+  pa.B = c.B;
+  pa.E = c.E;
+  pa.strideS = 8192;
+  pa.seed = c.seed;
+  pa.mlp = c.mode == kModeLearned ? learned_weights_device(device) : nullptr;
+  pa.setup = setup_in_plan0 ? 1u : 0u;
+  pa.nk = nk;
+  pa.ctl_words = ctl_words;
+  pa.ks = d_ks;
+  pa.k_rank = k_rank;
+  pa.k_state = k_state;
+  pa.bad = dbad;
+  pa.c0 = c0;
+  return pa;
+}
+
+// Plan-kernel arguments that do not change from pass to pass.
+template <class K>
+static PlanArgs<K> plan_args_common(const Config& c, const Layout& L, char* W, uint32_t nk, void* d_out) {
+  auto P = [&](int b) { return (void*)(W + L.off[b]); };
+  PlanArgs<K> pl{};
+  pl.totals = (uint32_t*)P(bTotals);
+  pl.nk = nk;
+  pl.k_rank = (uint32_t*)P(bKRank);
+  pl.k_state = (uint32_t*)P(bKState);
+  pl.k_aux = (uint32_t*)P(bKAux);
+  pl.slot_dst = (uint32_t*)P(bSlotDst);
+  pl.slot_bucket = (uint32_t*)P(bSlotBucket);
+  pl.act_bucket = (uint32_t*)P(bActBucket);
+  pl.act_cls = (uint32_t*)P(bActCls);
+  pl.act_cnt = (uint32_t*)P(bActCnt);
+  pl.act_dst = (uint32_t*)P(bActDst);
+  pl.act_slot = (uint32_t*)P(bActSlot);
+  pl.act_klo = (K*)P(bActKlo);
+  pl.act_khi = (K*)P(bActKhi);
+  pl.small_start = (uint32_t*)P(bSmallStart);
+  pl.small_len = (uint32_t*)P(bSmallLen);
+  pl.small_klo = (K*)P(bSmallKlo);
+  pl.small_khi = (K*)P(bSmallKhi);
+  pl.small_kstart = (uint32_t*)P(bSmallKStart);
+  pl.small_order = (uint32_t*)P(bSmallOrder);
+  pl.klist = (uint32_t*)P(bKList);
+  pl.out = d_out;
+  pl.fin_cap = c.fin_cap;
+  pl.B_req = c.B;
+  pl.B_cap = c.B_cap;
+  pl.G_max = c.G_max_next;
+  pl.min_tile = c.min_tile;
+  return pl;
+}
+
+// Dynamic LDS the pass-1 kernels are given in a schedule fixed before the
+// first pass has run (bounds instead of the first pass's exact table sizes):
+// their tables without the pass-1 range parameters, and at least the 112 KB
+// up to which the parameters join them (the kernels decide that on the device).
+template <class K>
+static size_t pass1_lds_fixed(const Config& c, size_t own) {
+  const size_t base = own + pass1_map_bytes<K>(c.B0_bound, c.R0_bound, LutGeom<K>::L2TABLES, 0);
+  return std::max<size_t>(base, 112 * 1024);
+}
+
+// The whole call enqueued on `st` with no host reads between kernels (a
+// device-driven schedule): every pass's kernels are launched with shapes and
+// LDS sized for the largest pass the call can produce; a kernel whose pass
+// turned out empty (its control block still zero) or that has nothing to do
+// in it returns at once.  The last of c.max_passes passes finishes every
+// k-bucket left, whatever its size.  flags: device-visible words
+// [0] finish error, [1] a k outside [1, n], [2] NaNs in the input -- the
+// caller reads them after the stream has completed.  Capturable in a hipGraph
+// (no host synchronisation, no allocation, no memset).
+template <class T>
+static ggms_status launch_fixed(const Config& c, const Layout& L, char* W, uint32_t* flags, const uint32_t* d_ks,
+                                T* d_out, int device, int cus, const T* d_vec, uint32_t n, uint32_t nk,
+                                hipStream_t st, Prof& prof) {
+  using K = typename KeyTraits<T>::K;
+  auto P = [&](int b) { return (void*)(W + L.off[b]); };
+  K* bufA = (K*)P(bBufA);
+  K* bufB = (K*)P(bBufB);
+  uint32_t* rows = (uint32_t*)P(bRows);
+  uint32_t* totals = (uint32_t*)P(bTotals);
+  Range<K>* rng[2] = {(Range<K>*)P(bRanges0), (Range<K>*)P(bRanges1)};
+  Range<K>* rng_first = (Range<K>*)P(bRangesFirst);
+  PassCtl* ctl = (PassCtl*)P(bCtl);
+  uint32_t* k_rank = (uint32_t*)P(bKRank);
+  uint32_t* k_state = (uint32_t*)P(bKState);
+  uint32_t* klist = (uint32_t*)P(bKList);
+  const size_t ctlb = sizeof(PassCtl);
+
+  PassCtl c0{};
+  c0.n = n;
+  uint32_t tile0 = 0, G0 = 0;
+  first_pass_tiling(n, c.G_max, c.min_tile, &tile0, &G0);
+  c0.tile = tile0;
+  c0.G = G0;
+  c0.nan_lo_bucket = kNone32;
+  c0.nan_hi_bucket = kNone32;
+  // the status words start clear (a fill kernel: capturable, unlike a memset)
+  k_fill_u32<<<1, 64, 0, st>>>(flags, 3, 0u);
+  HIP_TRY(hipGetLastError());
+  const uint32_t ctl_words = (uint32_t)((c.max_passes + 2) * ctlb / 4);
+  const bool setup_in_plan0 = n > c.fin_cap && nk <= kSetupInPlan0MaxKs;
+  if (!setup_in_plan0) {
+    prof.begin(GGMS_KERNEL_OTHER, -1);
+    const uint32_t threads = std::max(ctl_words, nk);
+    k_setup<<<(threads + 255) / 256, 256, 0, st>>>(d_ks, nk, n, k_rank, k_state, ctl, ctl_words, c0, flags + 1);
+    prof.end();
+    HIP_TRY(hipGetLastError());
+  }
+  const uint32_t fin_grid = std::max<uint32_t>(1, std::min<uint32_t>(4 * (uint32_t)cus, 65535));
+  const size_t fin_lds = (size_t)(c.fin_cap / 2) * sizeof(K);
+  set_lds_attr((const void*)k_finish<T>, fin_lds);
+  if (n <= c.fin_cap) {
+    const bool wide = finish_wide_segment<K>(n, (K)0, KeyMax<K>::v, c.fin_cap);
+    prof.begin(GGMS_KERNEL_OTHER, -1);
+    k_tiny_setup<T><<<1, 1024, 0, st>>>(d_vec, n, nk, bufA, k_rank, k_state, (uint32_t*)P(bSmallStart),
+                                        (uint32_t*)P(bSmallLen), (K*)P(bSmallKlo), (K*)P(bSmallKhi),
+                                        (uint32_t*)P(bSmallKStart), klist, ctl, wide ? 1u : 0u);
+    prof.end();
+    HIP_TRY(hipGetLastError());
+    prof.begin(GGMS_KERNEL_FINISH, 0);
+    k_finish<T><<<1, 1024, fin_lds, st>>>(
+        bufA, ctl, (uint32_t*)P(bSmallStart), (uint32_t*)P(bSmallLen), (K*)P(bSmallKlo), (K*)P(bSmallKhi),
+        (uint32_t*)P(bSmallKStart), nullptr, klist, k_rank, k_state, d_out, c.fin_cap, flags);
+    if (wide)
+      k_finish_wide<T><<<1, 1024, 0, st>>>(bufA, bufB, ctl, (uint32_t*)P(bSmallStart), (uint32_t*)P(bSmallLen),
+                                           (K*)P(bSmallKlo), (K*)P(bSmallKhi), (uint32_t*)P(bSmallKStart), nullptr,
+                                           klist, k_rank, k_state, d_out, c.fin_cap);
+    prof.end();
+    HIP_TRY(hipGetLastError());
+    return GGMS_OK;
+  }
+  const Plan0Args pa = plan0_args<T>(c, device, nk, ctl_words, setup_in_plan0, d_ks, k_rank, k_state, flags + 1, c0);
+  const size_t lds_p0 = plan0_lds_bytes<K>(pow2ceil_host(c.S));
+  LDS_CHECK(lds_p0);
+  set_lds_attr((const void*)k_plan0<T>, lds_p0);
+  prof.begin(GGMS_KERNEL_PLAN0, 0);
+  k_plan0<T><<<1, 1024, lds_p0, st>>>(d_vec, pa, rng_first, (K*)P(bLut), ctl);
+  prof.end();
+  HIP_TRY(hipGetLastError());
+
+  PlanArgs<K> pl = plan_args_common<K>(c, L, W, nk, d_out);
+  uint16_t* slot_tab[2] = {(uint16_t*)P(bSlotOfBucket),
+                           (uint16_t*)P(bSlotOfBucket) + (std::max<size_t>(c.B0_bound, c.B_cap) + 64)};
+  const uint32_t NT = c.NT;
+  const uint32_t Gn = c.G_max_next;
+  const uint32_t U_slot_bound = std::min<uint32_t>(nk, std::max(c.B0_bound, c.B_cap));
+  const uint32_t passes = c.one_pass ? 1u : c.max_passes;
+  for (uint32_t pass = 0; pass < passes; ++pass) {
+    PassCtl* cp = ctl + pass;
+    Range<K>* rc = pass == 0 ? rng_first : rng[pass & 1];
+    K* src = (pass & 1) ? bufA : bufB;
+    K* dst = (pass & 1) ? bufB : bufA;
+    uint32_t* wgf_cur = (uint32_t*)P(bWgFirst) + (size_t)(pass & 1) * c.G_max;
+    const uint32_t B_bound = pass == 0 ? c.B0_bound : c.B_cap;
+    // count
+    if (pass == 0) {
+      size_t lds = count_lds0<T>(c);
+      LDS_CHECK(count_lds0_total<T>(c));
+      const size_t hist2 = (size_t)c.B0_bound * 4;
+      const uint32_t tpw =
+          (count_lds0_total<T>(c) > kLdsOneWgPerCu && count_lds0_total<T>(c) + hist2 <= kLdsLimit) ? 2u : 1u;
+      if (tpw == 2) lds += hist2;
+      set_lds_attr((const void*)k_count0<T>, lds);
+      prof.begin(GGMS_KERNEL_COUNT0, (int)pass);
+      k_count0<T><<<(G0 + tpw - 1) / tpw, NT, lds, st>>>(d_vec, rc, (K*)P(bLut), cp, rows, tpw);
+      prof.end();
+    } else if (pass == 1) {
+      const size_t lds = pass1_lds_fixed<K>(c, ((size_t)B_bound * 4 + 15) / 16 * 16);
+      LDS_CHECK(lds);
+      set_lds_attr((const void*)k_count1<K>, lds);
+      prof.begin(GGMS_KERNEL_COUNTS, (int)pass);
+      k_count1<K><<<Gn, NT, lds, st>>>(src, ctl, (K*)P(bLut), rng_first, (uint64_t*)P(bRank0),
+                                        (MapParam<K>*)P(bParams1), 112u * 1024u, cp, rows);
+      prof.end();
+    } else {
+      const size_t lds = (size_t)B_bound * 4;
+      set_lds_attr((const void*)k_countS<K>, lds);
+      prof.begin(GGMS_KERNEL_COUNTS, (int)pass);
+      k_countS<K><<<Gn, NT, lds, st>>>(src, rc, wgf_cur, cp, rows);
+      prof.end();
+    }
+    HIP_TRY(hipGetLastError());
+    prof.begin(GGMS_KERNEL_COLSUM, (int)pass);
+    k_colsum<<<(B_bound + 63) / 64, 1024, 0, st>>>(rows, cp, totals);
+    prof.end();
+    HIP_TRY(hipGetLastError());
+    // plan
+    pl.ranges = rc;
+    pl.ctl = cp;
+    pl.ctl_next = ctl + pass + 1;
+    pl.next_ranges = rng[(pass + 1) & 1];
+    pl.slot_of_bucket = slot_tab[pass & 1];
+    pl.wg_first_next = (uint32_t*)P(bWgFirst) + (size_t)((pass + 1) & 1) * c.G_max;
+    pl.force_finish = (c.one_pass || pass + 1 >= passes) ? 1u : 0u;
+    pl.merge_small = (pass == 0 && !c.one_pass) ? 1u : 0u;
+    pl.next_params = (pass == 0 && !c.one_pass) ? (MapParam<K>*)P(bParams1) : nullptr;
+    pl.rank_words = (pass == 0 && !c.one_pass) ? (uint64_t*)P(bRank0) : nullptr;
+    pl.mirror = nullptr;
+    pl.nan_out = pass == 0 ? flags + 2 : nullptr;
+    const size_t lds_plan = (size_t)B_bound * 8 + (size_t)rank_word_count(B_bound) * 8;
+    LDS_CHECK(lds_plan);
+    set_lds_attr((const void*)k_plan<T>, lds_plan);
+    prof.begin(GGMS_KERNEL_PLAN, (int)pass);
+    k_plan<T><<<1, 1024, lds_plan, st>>>(pl);
+    prof.end();
+    HIP_TRY(hipGetLastError());
+    // compaction
+    if (pass == 0 && c.one_pass) {
+      const size_t lds = Pass0Map<K>::param_bytes(c.R0_bound) + compact_lds_bytes(B_bound, U_slot_bound + 4);
+      LDS_CHECK(lds + LutGeom<K>::bytes());
+      set_lds_attr((const void*)k_compact0_seg<T>, lds);
+      prof.begin(GGMS_KERNEL_COMPACT0, (int)pass);
+      k_compact0_seg<T><<<G0, NT, lds, st>>>(d_vec, rc, (K*)P(bLut), cp, rows, pl.slot_of_bucket, pl.slot_dst,
+                                             pl.slot_bucket, dst);
+      prof.end();
+    } else if (pass == 0) {
+      const size_t lds = (size_t)c.R0_bound * sizeof(MapParam<K>) + (size_t)B_bound * 2;
+      LDS_CHECK(lds + LutGeom<K>::bytes());
+      set_lds_attr((const void*)k_compact0<T>, lds);
+      prof.begin(GGMS_KERNEL_COMPACT0, (int)pass);
+      k_compact0<T><<<G0, NT, lds, st>>>(d_vec, rc, (K*)P(bLut), cp, rows, pl.slot_of_bucket, pl.slot_bucket, dst);
+      prof.end();
+    } else if (pass == 1) {
+      const size_t lds = pass1_lds_fixed<K>(c, compact_lds_bytes(B_bound, U_slot_bound + 4));
+      LDS_CHECK(lds);
+      set_lds_attr((const void*)k_compact1<K>, lds);
+      prof.begin(GGMS_KERNEL_COMPACTS, (int)pass);
+      k_compact1<K><<<Gn, NT, lds, st>>>(src, ctl, (K*)P(bLut), rng_first, (uint64_t*)P(bRank0),
+                                          (MapParam<K>*)P(bParams1), 112u * 1024u, cp, rows, pl.slot_of_bucket,
+                                          pl.slot_dst, pl.slot_bucket, dst);
+      prof.end();
+    } else {
+      const size_t lds = compact_lds_bytes(B_bound, U_slot_bound + 4);
+      LDS_CHECK(lds);
+      set_lds_attr((const void*)k_compactS<K>, lds);
+      prof.begin(GGMS_KERNEL_COMPACTS, (int)pass);
+      k_compactS<K><<<Gn, NT, lds, st>>>(src, rc, wgf_cur, cp, rows, pl.slot_of_bucket, pl.slot_dst,
+                                          pl.slot_bucket, dst);
+      prof.end();
+    }
+    HIP_TRY(hipGetLastError());
+    // finish (the iterative first pass hands every k-bucket to pass 1)
+    if (pass > 0 || c.one_pass) {
+      prof.begin(GGMS_KERNEL_FINISH, (int)pass);
+      k_finish<T><<<fin_grid, 1024, fin_lds, st>>>(dst, cp, pl.small_start, pl.small_len, pl.small_klo,
+                                                    pl.small_khi, pl.small_kstart, pl.small_order, klist, k_rank,
+                                                    k_state, d_out, c.fin_cap, flags);
+      k_finish_wide<T><<<(uint32_t)cus, 1024, 0, st>>>(dst, src, cp, pl.small_start, pl.small_len, pl.small_klo,
+                                                        pl.small_khi, pl.small_kstart, pl.small_order, klist, k_rank,
+                                                        k_state, d_out, c.fin_cap);
+      prof.end();
+      HIP_TRY(hipGetLastError());
+    }
+  }
+  return GGMS_OK;
+}
+
+// Status of a device-driven call from its status words.
+static ggms_status fixed_status(const uint32_t* f) {
+  if (f[1]) return GGMS_EINVAL;
+  if (f[2]) return GGMS_ENAN;
+  if (f[0]) return GGMS_EDEVICE;
+  return GGMS_OK;
+}
+
 template <class T>
 static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint64_t n64, const uint32_t* ks,
                               bool ks_on_device, uint32_t nk, void* out, bool out_on_device, const ggms_opts& o) {
   using K = typename KeyTraits<T>::K;
-  if (n64 == 0 || n64 > 0xFFFFFFFFull) return GGMS_EINVAL;
+  if (n64 == 0 || n64 > kMaxN) return GGMS_EINVAL;
   if (nk == 0) return GGMS_OK;
   if (!d_vec || !ks || !out) return GGMS_EINVAL;
   const uint32_t n = (uint32_t)n64;
@@ -328,11 +640,20 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
       if (ks[i] < 1 || ks[i] > n) return GGMS_EINVAL;
 
   Layout L = make_layout<T>(n64, nk, c);
-  ggms_status s = ensure_ws(ctx, L.total);
-  if (s) return s;
   const size_t ctlb = sizeof(PassCtl);
   const size_t ks_bytes = ((size_t)nk * 4 + 255) / 256 * 256;
-  s = ensure_pinned(ctx, 6 * ctlb + 256 + ks_bytes + (size_t)nk * sizeof(T) + 4096);
+  const size_t pinned_need = 6 * ctlb + 256 + ks_bytes + (size_t)nk * sizeof(T) + 4096;
+  // under hipGraph capture only the asynchronous device-driven form can run,
+  // and it must find its memory ready (ggms_ctx_reserve): nothing allocates
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(st, &cap) != hipSuccess) cap = hipStreamCaptureStatusNone;
+  if (cap != hipStreamCaptureStatusNone) {
+    if (!c.device_schedule || !ks_on_device || !out_on_device || o.profile) return GGMS_EINVAL;
+    if (ctx->ws.bytes < L.total || ctx->pinned_bytes < pinned_need) return GGMS_ENOMEM;
+  }
+  ggms_status s = ensure_ws(ctx, L.total);
+  if (s) return s;
+  s = ensure_pinned(ctx, pinned_need);
   if (s) return s;
   char* W = (char*)ctx->ws.base;
   auto P = [&](int b) { return (void*)(W + L.off[b]); };
@@ -369,21 +690,29 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
   *hflag = 0;
   *hbad = 0;
   if (!ks_on_device) memcpy(hks, ks, (size_t)nk * 4);
+  ggms_stats& stt = ctx->stats;
+  memset(&stt, 0, sizeof(stt));
+  Prof prof(ctx, st, o.profile != 0);
+
+  if (c.device_schedule) {
+    s = launch_fixed<T>(c, L, W, dflag, d_ks, d_out, ctx->device, ctx->cus, d_vec, n, nk, st, prof);
+    if (s) return s;
+    if (out_on_device && ks_on_device) return GGMS_OK;  // asynchronous: ggms_ctx_device_status after the stream
+    HIP_TRY(hipStreamSynchronize(st));
+    if (!out_on_device) memcpy(out, hout, (size_t)nk * sizeof(T));
+    prof.collect(stt);
+    return fixed_status((const uint32_t*)hflag);
+  }
 
   // control block 0
   PassCtl c0{};
   c0.n = n;
-  uint32_t G0 = (uint32_t)std::min<uint64_t>(c.G_max, (n64 + c.min_tile - 1) / c.min_tile);
-  if (G0 < 1) G0 = 1;
-  uint32_t tile0 = (uint32_t)(((n64 + G0 - 1) / G0 + 63) & ~63ull);
-  G0 = (uint32_t)((n64 + tile0 - 1) / tile0);
+  uint32_t tile0 = 0, G0 = 0;
+  first_pass_tiling(n64, c.G_max, c.min_tile, &tile0, &G0);
   c0.tile = tile0;
   c0.G = G0;
   c0.nan_lo_bucket = kNone32;
   c0.nan_hi_bucket = kNone32;
-  ggms_stats& stt = ctx->stats;
-  memset(&stt, 0, sizeof(stt));
-  Prof prof(ctx, st, o.profile != 0);
   // setup (control blocks, ks -> ranks): its own kernel, or folded into the
   // first-pass planning kernel when the ks are few enough for one workgroup
   const uint32_t ctl_words = (uint32_t)((c.max_passes + 2) * ctlb / 4);
@@ -413,7 +742,7 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
         bufA, ctl, (uint32_t*)P(bSmallStart), (uint32_t*)P(bSmallLen), (K*)P(bSmallKlo), (K*)P(bSmallKhi),
         (uint32_t*)P(bSmallKStart), nullptr, klist, k_rank, k_state, d_out, c.fin_cap, dflag);
     if (wide)
-      k_finish_wide<T><<<1, 1024, 0, st>>>(bufA, ctl, (uint32_t*)P(bSmallStart), (uint32_t*)P(bSmallLen),
+      k_finish_wide<T><<<1, 1024, 0, st>>>(bufA, bufB, ctl, (uint32_t*)P(bSmallStart), (uint32_t*)P(bSmallLen),
                                            (K*)P(bSmallKlo), (K*)P(bSmallKhi), (uint32_t*)P(bSmallKStart), nullptr,
                                            klist, k_rank, k_state, d_out, c.fin_cap);
     prof.end();
@@ -425,23 +754,7 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
     stt.finished_segments = 1;
   } else {
     // first-pass splitter
-    Plan0Args pa{};
-    pa.mode = c.mode;
-    pa.S = c.S;
-    pa.P = c.P;
-    pa.B = c.B;
-    pa.E = c.E;
-    pa.strideS = 8192;
-    pa.seed = c.seed;
-    pa.mlp = c.mode == kModeLearned ? learned_weights_device(ctx->device) : nullptr;
-    pa.setup = setup_in_plan0 ? 1u : 0u;
-    pa.nk = nk;
-    pa.ctl_words = ctl_words;
-    pa.ks = d_ks;
-    pa.k_rank = k_rank;
-    pa.k_state = k_state;
-    pa.bad = dbad;
-    pa.c0 = c0;
+    const Plan0Args pa = plan0_args<T>(c, ctx->device, nk, ctl_words, setup_in_plan0, d_ks, k_rank, k_state, dbad, c0);
     size_t lds_p0 = plan0_lds_bytes<K>(pow2ceil_host(c.S));
     LDS_CHECK(lds_p0);
     set_lds_attr((const void*)k_plan0<T>, lds_p0);
@@ -497,7 +810,7 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
         set_lds_attr((const void*)k_count1<K>, lds);
         prof.begin(GGMS_KERNEL_COUNTS, (int)pass);
         k_count1<K><<<G, NT, lds, st>>>(src, ctl, (K*)P(bLut), rng_first, (uint64_t*)P(bRank0),
-                                        (MapParam<K>*)P(bParams1), p1_lds, cp, rows);
+                                        (MapParam<K>*)P(bParams1), p1_lds ? (uint32_t)lds : 0u, cp, rows);
         prof.end();
       } else {
         size_t lds = (size_t)B_bound * 4;
@@ -513,45 +826,19 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
       prof.end();
       HIP_TRY(hipGetLastError());
       // plan
-      PlanArgs<K> pl{};
-      pl.totals = totals;
+      PlanArgs<K> pl = plan_args_common<K>(c, L, W, nk, d_out);
       pl.ranges = rc;
       pl.ctl = cp;
       pl.ctl_next = cn;
       pl.next_ranges = rn;
-      pl.nk = nk;
-      pl.k_rank = k_rank;
-      pl.k_state = k_state;
-      pl.k_aux = k_aux;
       pl.slot_of_bucket = slot_tab[pass & 1];
-      pl.slot_dst = (uint32_t*)P(bSlotDst);
-      pl.slot_bucket = (uint32_t*)P(bSlotBucket);
-      pl.act_bucket = (uint32_t*)P(bActBucket);
-      pl.act_cls = (uint32_t*)P(bActCls);
-      pl.act_cnt = (uint32_t*)P(bActCnt);
-      pl.act_dst = (uint32_t*)P(bActDst);
-      pl.act_slot = (uint32_t*)P(bActSlot);
-      pl.act_klo = (K*)P(bActKlo);
-      pl.act_khi = (K*)P(bActKhi);
-      pl.small_start = (uint32_t*)P(bSmallStart);
-      pl.small_len = (uint32_t*)P(bSmallLen);
-      pl.small_klo = (K*)P(bSmallKlo);
-      pl.small_khi = (K*)P(bSmallKhi);
-      pl.small_kstart = (uint32_t*)P(bSmallKStart);
-      pl.small_order = (uint32_t*)P(bSmallOrder);
-      pl.klist = klist;
       pl.wg_first_next = wgf_next;
-      pl.out = d_out;
-      pl.fin_cap = c.fin_cap;
-      pl.B_req = c.B;
-      pl.B_cap = c.B_cap;
-      pl.G_max = c.G_max_next;
-      pl.min_tile = c.min_tile;
-      pl.force_finish = (pass + 1 >= c.max_passes) ? 1u : c.force_finish;
-      pl.merge_small = pass == 0 ? 1u : 0u;
-      pl.next_params = pass == 0 ? (MapParam<K>*)P(bParams1) : nullptr;
-      pl.rank_words = pass == 0 ? (uint64_t*)P(bRank0) : nullptr;
+      pl.force_finish = (c.one_pass || pass + 1 >= c.max_passes) ? 1u : c.force_finish;
+      pl.merge_small = (pass == 0 && !c.one_pass) ? 1u : 0u;
+      pl.next_params = (pass == 0 && !c.one_pass) ? (MapParam<K>*)P(bParams1) : nullptr;
+      pl.rank_words = (pass == 0 && !c.one_pass) ? (uint64_t*)P(bRank0) : nullptr;
       pl.mirror = dmirror;
+      pl.nan_out = nullptr;
       size_t lds_plan = (size_t)B_bound * 8 + (size_t)rank_word_count(B_bound) * 8;
       LDS_CHECK(lds_plan);
       set_lds_attr((const void*)k_plan<T>, lds_plan);
@@ -561,7 +848,16 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
       HIP_TRY(hipGetLastError());
       HIP_TRY(hipEventRecord(ctx->plan_ev, st));
       // compaction
-      if (pass == 0) {
+      if (pass == 0 && c.one_pass) {
+        // every k-bucket into its own segment (the finish reads segments)
+        size_t lds = Pass0Map<K>::param_bytes(R_bound) + compact_lds_bytes(B_bound, U_slot_bound + 4);
+        LDS_CHECK(lds + LutGeom<K>::bytes());
+        set_lds_attr((const void*)k_compact0_seg<T>, lds);
+        prof.begin(GGMS_KERNEL_COMPACT0, (int)pass);
+        k_compact0_seg<T><<<G, NT, lds, st>>>(d_vec, rc, (K*)P(bLut), cp, rows, pl.slot_of_bucket, pl.slot_dst,
+                                              pl.slot_bucket, dst);
+        prof.end();
+      } else if (pass == 0) {
         // dynamic part: map parameters then the slot table (cell table is static LDS)
         size_t lds = (size_t)R_bound * sizeof(MapParam<K>) + (size_t)B_bound * 2;
         LDS_CHECK(lds + LutGeom<K>::bytes());
@@ -580,7 +876,7 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
         set_lds_attr((const void*)k_compact1<K>, lds);
         prof.begin(GGMS_KERNEL_COMPACTS, (int)pass);
         k_compact1<K><<<G, NT, lds, st>>>(src, ctl, (K*)P(bLut), rng_first, (uint64_t*)P(bRank0),
-                                          (MapParam<K>*)P(bParams1), p1_lds, cp, rows,
+                                          (MapParam<K>*)P(bParams1), p1_lds ? (uint32_t)lds : 0u, cp, rows,
                                           pl.slot_of_bucket, pl.slot_dst, pl.slot_bucket, dst);
         prof.end();
       } else {
@@ -598,7 +894,8 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
       HIP_TRY(hipEventSynchronize(ctx->plan_ev));
       const PassCtl h = hmirror[0];
       const PassCtl hn = hmirror[1];
-      if (pass > 0 && h.U_small > 0) {  // the first pass hands every k-bucket to pass 1
+      // the iterative first pass hands every k-bucket to pass 1
+      if ((pass > 0 || c.one_pass) && h.U_small > 0) {
         // every segment within a quarter of the capacity: half-size workgroups
         // (512 threads, half the LDS) so twice as many run at once
         const bool half = h.small_max <= c.fin_cap / 4;
@@ -608,7 +905,7 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
             k_rank, k_state, d_out, half ? c.fin_cap / 2 : c.fin_cap, dflag);
         if (h.U_wide > 0)
           k_finish_wide<T><<<std::min<uint32_t>(h.U_wide, (uint32_t)ctx->cus), 1024, 0, st>>>(
-              dst, cp, pl.small_start, pl.small_len, pl.small_klo, pl.small_khi, pl.small_kstart, pl.small_order,
+              dst, src, cp, pl.small_start, pl.small_len, pl.small_klo, pl.small_khi, pl.small_kstart, pl.small_order,
               klist, k_rank, k_state, d_out, c.fin_cap);
         prof.end();
         HIP_TRY(hipGetLastError());
@@ -653,7 +950,7 @@ static ggms_status run_first_pass(ggms_ctx* ctx, const T* d_vec, uint64_t n64, c
                                   ggms_range_u64* ranges_host, uint32_t* num_ranges, uint32_t* hist_host,
                                   uint32_t* num_buckets) {
   using K = typename KeyTraits<T>::K;
-  if (n64 == 0 || n64 > 0xFFFFFFFFull || !d_vec || !num_ranges || !num_buckets) return GGMS_EINVAL;
+  if (n64 == 0 || n64 > kMaxN || !d_vec || !num_ranges || !num_buckets) return GGMS_EINVAL;
   const uint32_t n = (uint32_t)n64;
   hipStream_t st = (hipStream_t)o.stream;
   Config c = resolve_config<K>(GGMS_ALG_ITERATIVE, n64, 1, o, ctx->cus);
@@ -676,10 +973,8 @@ static ggms_status run_first_pass(ggms_ctx* ctx, const T* d_vec, uint64_t n64, c
   char* hdata = H + 4 * ctlb;
   PassCtl c0{};
   c0.n = n;
-  uint32_t G0 = (uint32_t)std::min<uint64_t>(c.G_max, (n64 + c.min_tile - 1) / c.min_tile);
-  if (G0 < 1) G0 = 1;
-  uint32_t tile0 = (uint32_t)(((n64 + G0 - 1) / G0 + 63) & ~63ull);
-  G0 = (uint32_t)((n64 + tile0 - 1) / tile0);
+  uint32_t tile0 = 0, G0 = 0;
+  first_pass_tiling(n64, c.G_max, c.min_tile, &tile0, &G0);
   c0.tile = tile0;
   c0.G = G0;
   c0.nan_lo_bucket = kNone32;
@@ -751,17 +1046,41 @@ static ggms_ctx* default_ctx() {
   return g_default[dev];
 }
 
+// Status of the calling thread's last reference-shaped wrapper call.
+static ggms_status& wrapper_status() {
+  static thread_local ggms_status s = GGMS_OK;
+  return s;
+}
+
 template <class T>
 static int wrapper(ggms_alg alg, ggms_dtype dt, T* d_vector, int length, const uint32_t* kVals_ori, int numKs,
                    T* outputs, int blocks, int threads) {
   (void)blocks;
-  if (length <= 0 || numKs < 0) return GGMS_EINVAL;
-  ggms_ctx* c = default_ctx();
-  if (!c) return GGMS_EDEVICE;
-  ggms_opts o;
-  ggms_opts_default(&o);
-  if (threads > 0 && threads % 64 == 0 && threads <= 1024) o.block_threads = (uint32_t)threads;
-  return ggms_select(c, alg, dt, d_vector, (uint64_t)length, kVals_ori, (uint32_t)numKs, outputs, &o);
+  ggms_status s = GGMS_OK;
+  ggms_ctx* c = nullptr;
+  if (length <= 0 || numKs < 0) s = GGMS_EINVAL;
+  else if (!(c = default_ctx())) s = GGMS_EDEVICE;
+  else {
+    ggms_opts o;
+    ggms_opts_default(&o);
+    if (threads > 0 && threads % 64 == 0 && threads <= 1024) o.block_threads = (uint32_t)threads;
+    s = ggms_select(c, alg, dt, d_vector, (uint64_t)length, kVals_ori, (uint32_t)numKs, outputs, &o);
+  }
+  wrapper_status() = s;
+  return s;
+}
+
+template <class T>
+static ggms_status reserve_t(ggms_ctx* ctx, uint64_t n, uint32_t nk, const ggms_opts& o) {
+  using K = typename KeyTraits<T>::K;
+  if (n == 0 || n > kMaxN) return GGMS_EINVAL;
+  const Config c = resolve_config<K>(GGMS_ALG_ITERATIVE, n, nk, o, ctx->cus);
+  const Config cb = resolve_config<K>(GGMS_ALG_BUCKET, n, nk, o, ctx->cus);
+  const Layout L = make_layout<T>(n, nk, c), Lb = make_layout<T>(n, nk, cb);
+  ggms_status s = ensure_ws(ctx, std::max(L.total, Lb.total));
+  if (s) return s;
+  const size_t ks_bytes = ((size_t)nk * 4 + 255) / 256 * 256;
+  return ensure_pinned(ctx, 6 * sizeof(PassCtl) + 256 + ks_bytes + (size_t)nk * sizeof(T) + 4096);
 }
 
 }  // namespace ggms
@@ -785,6 +1104,8 @@ void ggms_opts_default(ggms_opts* o) {
   o->seed = 0x5EEDull;
   o->stream = nullptr;
 }
+
+ggms_status ggms_last_wrapper_status(void) { return wrapper_status(); }
 
 const char* ggms_status_str(ggms_status s) {
   switch (s) {
@@ -835,9 +1156,39 @@ void ggms_ctx_destroy(ggms_ctx* c) {
   delete c;
 }
 
+ggms_status ggms_ctx_reserve(ggms_ctx* ctx, ggms_dtype t, uint64_t n, uint32_t nk, const ggms_opts* opts) {
+  if (!ctx) return GGMS_EINVAL;
+  ggms_opts o;
+  if (opts) o = *opts; else ggms_opts_default(&o);
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  int cur = 0;
+  (void)hipGetDevice(&cur);
+  if (cur != ctx->device) (void)hipSetDevice(ctx->device);
+  ggms_status s = GGMS_EINVAL;
+  switch (t) {
+    case GGMS_F32: s = reserve_t<float>(ctx, n, nk, o); break;
+    case GGMS_F64: s = reserve_t<double>(ctx, n, nk, o); break;
+    case GGMS_U32: s = reserve_t<uint32_t>(ctx, n, nk, o); break;
+    case GGMS_I32: s = reserve_t<int32_t>(ctx, n, nk, o); break;
+  }
+  if (cur != ctx->device) (void)hipSetDevice(cur);
+  return s;
+}
+
+ggms_status ggms_ctx_device_status(const ggms_ctx* ctx) {
+  if (!ctx || !ctx->pinned) return GGMS_EINVAL;
+  return fixed_status((const uint32_t*)((const char*)ctx->pinned + 6 * sizeof(PassCtl)));
+}
+
 ggms_status ggms_ctx_stats(const ggms_ctx* c, ggms_stats* out) {
   if (!c || !out) return GGMS_EINVAL;
   *out = c->stats;
+  return GGMS_OK;
+}
+
+ggms_status ggms_first_pass_tiling(uint64_t n, uint32_t workgroups, uint32_t* tile, uint32_t* groups) {
+  if (n == 0 || n > kMaxN || workgroups == 0 || !tile || !groups) return GGMS_EINVAL;
+  first_pass_tiling(n, std::min<uint32_t>(workgroups, kMaxWorkgroups), kMinTile, tile, groups);
   return GGMS_OK;
 }
 
@@ -858,7 +1209,7 @@ static ggms_status select_common(ggms_ctx* ctx, ggms_alg alg, ggms_dtype t, cons
                                  const uint32_t* ks, bool ks_dev, uint32_t nk, void* out, bool out_dev,
                                  const ggms_opts* opts) {
   if (!ctx) return GGMS_EINVAL;
-  if ((unsigned)alg > 2u) return GGMS_EINVAL;
+  if ((unsigned)alg > 3u) return GGMS_EINVAL;
   ggms_opts o;
   if (opts) o = *opts; else ggms_opts_default(&o);
   std::lock_guard<std::mutex> lk(ctx->mu);
@@ -885,6 +1236,21 @@ ggms_status ggms_select(ggms_ctx* ctx, ggms_alg alg, ggms_dtype t, const void* d
 ggms_status ggms_select_device(ggms_ctx* ctx, ggms_alg alg, ggms_dtype t, const void* d_vec, uint64_t n,
                                const uint32_t* d_ks, uint32_t nk, void* d_out, const ggms_opts* opts) {
   return select_common(ctx, alg, t, d_vec, n, d_ks, true, nk, d_out, true, opts);
+}
+
+ggms_status ggms_select_topk(ggms_ctx* ctx, ggms_alg alg, ggms_dtype t, const void* d_vec, uint64_t n, uint32_t K,
+                             void* out_host, const ggms_opts* opts) {
+  if (!ctx || !d_vec || n == 0 || n > kMaxN) return GGMS_EINVAL;
+  if (K == 0) return GGMS_OK;
+  if (K > n || !out_host) return GGMS_EINVAL;
+  std::vector<uint32_t> ks;
+  try {
+    ks.resize(K);
+  } catch (const std::bad_alloc&) {
+    return GGMS_ENOMEM;
+  }
+  for (uint32_t i = 0; i < K; ++i) ks[i] = i + 1;  // k = 1 .. K: the K largest, largest first
+  return select_common(ctx, alg, t, d_vec, n, ks.data(), false, K, out_host, false, opts);
 }
 
 ggms_status ggms_debug_first_pass(ggms_ctx* ctx, ggms_dtype t, const void* d_vec, uint64_t n, const ggms_opts* opts,
@@ -917,6 +1283,12 @@ ggms_status ggms_debug_first_pass(ggms_ctx* ctx, ggms_dtype t, const void* d_vec
     (void)rank;                                                                                                 \
     return wrapper<TY>(GGMS_ALG_DISTRIBUTED_1GPU, DT, d_vector, length, kVals_ori, numKs, outputs, blocks,      \
                        threads);                                                                                \
+  }                                                                                                             \
+  int ggms_distributedBucketMultiselectWrapper_##SUF(TY* d_vector, int length, const uint32_t* kVals_ori,       \
+                                                     int numKs, TY* outputs, int blocks, int threads, int rank) {\
+    (void)rank;                                                                                                 \
+    return wrapper<TY>(GGMS_ALG_DISTRIBUTED_BUCKET_1GPU, DT, d_vector, length, kVals_ori, numKs, outputs,       \
+                       blocks, threads);                                                                        \
   }
 
 GGMS_WRAPPERS(f32, float, GGMS_F32)

@@ -474,7 +474,9 @@ __host__ __device__ __forceinline__ bool finish_fits_half(uint32_t len, K klo, K
 // Size classes of small segments for the finish order (1024 keys per class,
 // the largest first), then class 1: segments of at most kFinishWaveLen keys
 // and kFinishWaveKs ks, finished one per wave, then class 0, ordered last, for
-// the segments selected from global memory.  The one-wave class is used only
+// the segments selected from global memory (too wide and too large for the
+// LDS layouts, or larger than the finish capacity: the one-pass
+// bucketMultiselect finishes every k-bucket whatever its size).  The one-wave class is used only
 // when a pass leaves at least kFinishWaveMinSegs small segments: a wave
 // sweeping its segment from L2 is slower per segment than a workgroup with
 // the segment in LDS (2.6 K keys: ~35 vs ~12 us), and wins only once the
@@ -486,7 +488,7 @@ constexpr uint32_t kFinishWaveKs = 4;
 constexpr uint32_t kFinishWaveMinSegs = 2048;
 template <class K>
 __host__ __device__ __forceinline__ bool finish_wide_segment(uint32_t len, K klo, K khi, uint32_t fin_cap) {
-  return len <= fin_cap && !finish_fits_half<K>(len, klo, khi, fin_cap);
+  return len > fin_cap || !finish_fits_half<K>(len, klo, khi, fin_cap);
 }
 // The part of a segment's class that its length and key interval decide:
 // low 8 bits the class without the one-wave option, bit 8 set when its
@@ -1145,7 +1147,7 @@ __global__ void __launch_bounds__(1024) k_count0(const T* __restrict__ x, const 
   for (uint32_t h = 0; h < tiles_per_wg && w0 + h < G; ++h) {
     uint32_t* hist = hist0 + h * B;
     uint32_t t0 = (w0 + h) * tile;
-    uint32_t t1 = t0 + tile < n ? t0 + tile : n;
+    uint32_t t1 = tile_end(t0, tile, n);
     constexpr int N = Vec16<T>::N;
     stream_tile_vec_any<T, true>(
         x, t0, t1,
@@ -1201,7 +1203,7 @@ __global__ void __launch_bounds__(1024) k_countS(const K* __restrict__ buf, cons
   for (uint32_t i = threadIdx.x; i < B; i += blockDim.x) hist[i] = 0;
   __syncthreads();
   uint32_t t0 = blockIdx.x * tile;
-  uint32_t t1 = t0 + tile < n ? t0 + tile : n;
+  uint32_t t1 = tile_end(t0, tile, n);
   SegCursor<K> cur;
   cur.rg = ranges;
   cur.load(wg_first[blockIdx.x]);
@@ -1289,6 +1291,7 @@ template <class K> struct PlanArgs {
   MapParam<K>* next_params;  // packed map parameters of next_ranges (or null)
   uint64_t* rank_words;      // slot rank words of this pass's buckets (or null)
   PassCtl* mirror;           // host-visible copy of ctl and ctl_next, written last (or null)
+  uint32_t* nan_out;         // the first pass writes its NaN count here (or null)
 };
 
 template <class T>
@@ -1299,6 +1302,9 @@ __global__ void __launch_bounds__(1024) k_plan(PlanArgs<typename KeyTraits<T>::K
   __shared__ uint32_t sh_tot[8];
   const uint32_t nt = blockDim.x, t = threadIdx.x;
   const PassCtl& ctl = *a.ctl;
+  // a pass the previous plan left empty (its control block is still zero):
+  // nothing to plan, and the next pass stays empty too
+  if (ctl.n == 0) return;
   const uint32_t B = ctl.B;
   uint32_t* P = reinterpret_cast<uint32_t*>(smem);     // inclusive prefix of the histogram
   uint32_t* aidx = P + B;                              // active flag -> index + 1
@@ -1645,6 +1651,7 @@ __global__ void __launch_bounds__(1024) k_plan(PlanArgs<typename KeyTraits<T>::K
     if (ctl.nan_lo_bucket != kNone32) nan += a.totals[ctl.nan_lo_bucket];
     if (ctl.nan_hi_bucket != kNone32) nan += a.totals[ctl.nan_hi_bucket];
     cc->nan_count = nan;
+    if (a.nan_out) *a.nan_out = nan;
     // the host waits for this kernel, not for the compaction behind it: hand it
     // both control blocks (read back by the thread that wrote them)
     if (a.mirror) {
@@ -1722,7 +1729,7 @@ __global__ void __launch_bounds__(1024) k_compact0(const T* __restrict__ x, cons
   }
   __syncthreads();
   uint32_t t0 = blockIdx.x * tile;
-  uint32_t t1 = t0 + tile < n ? t0 + tile : n;
+  uint32_t t1 = tile_end(t0, tile, n);
   constexpr int N = Vec16<T>::N;
   stream_tile_vec_any<T, true>(
       x, t0, t1,
@@ -1765,6 +1772,67 @@ __global__ void __launch_bounds__(1024) k_compact0(const T* __restrict__ x, cons
           base = readlane_u32(base, first);
           if (act) out[base + mask_below(m)] = key;
         }
+      });
+}
+
+// First-pass compaction, segmented layout (the one-pass bucketMultiselect):
+// every k-bucket's elements land contiguously in its segment, as the
+// reference's copyElements does before sorting the remainder
+// (DistributedSMOS/bucketMultiselect.cu:779-793).  Workgroup w claims its run
+// of each segment with one global atomicAdd of its histogram-row count; each
+// survivor then takes its position with one LDS atomic.
+template <class T>
+__global__ void __launch_bounds__(1024) k_compact0_seg(const T* __restrict__ x,
+                                                       const Range<typename KeyTraits<T>::K>* __restrict__ ranges,
+                                                       const typename KeyTraits<T>::K* __restrict__ lut,
+                                                       const PassCtl* __restrict__ ctl, const uint32_t* __restrict__ rows,
+                                                       const uint16_t* __restrict__ slot_of_bucket,
+                                                       uint32_t* __restrict__ slot_dst,
+                                                       const uint32_t* __restrict__ slot_bucket,
+                                                       typename KeyTraits<T>::K* __restrict__ out) {
+  using K = typename KeyTraits<T>::K;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const uint32_t B = ctl->B, n = ctl->n, tile = ctl->tile, U_slots = ctl->U_slots;
+  if (blockIdx.x >= ctl->G || U_slots == 0) return;
+  __shared__ __align__(16) K s_lut[LutGeom<K>::kEntries];
+  Pass0Map<K> mp;
+  mp.lut = s_lut;
+  mp.set_params(smem, ctl->R);
+  CompactTables ct;
+  compact_init(ct, smem + Pass0Map<K>::param_bytes(ctl->R), B, U_slots, slot_of_bucket, slot_dst, slot_bucket,
+               rows + (size_t)blockIdx.x * B);
+  mp.load(lut, ranges, ctl->R);
+  __syncthreads();
+  const uint32_t t0 = blockIdx.x * tile;
+  const uint32_t t1 = tile_end(t0, tile, n);
+  constexpr int N = Vec16<T>::N;
+  stream_tile_vec_any<T, true>(
+      x, t0, t1,
+      [&](const T (&v)[N]) {
+        K k[N];
+#pragma unroll
+        for (int j = 0; j < N; ++j) k[j] = KeyTraits<T>::enc(v[j]);
+        uint32_t b[N];
+        mp.template buckets<N>(k, b);
+        uint32_t sl[N];
+        bool any = false;
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+          sl[j] = ct.slot[b[j]];
+          any |= sl[j] != kNone16;
+        }
+        if (wave_ballot(any)) {
+#pragma unroll
+          for (int j = 0; j < N; ++j)
+            if (sl[j] != kNone16) out[atomicAdd(&ct.cursor[sl[j]], 1u)] = k[j];
+        }
+      },
+      [&](T v) {
+        const K key = KeyTraits<T>::enc(v);
+        const uint32_t s = ct.slot[mp.bucket(key)];
+        const bool act = s != kNone16;
+        const uint32_t p = lds_claim(ct.cursor, s, act);
+        if (act) out[p] = key;
       });
 }
 
@@ -1833,23 +1901,33 @@ __device__ __forceinline__ void pass1_map_init(Pass1Map<K>& pm, unsigned char* p
   }
 }
 
+// The pass-1 kernels keep the pass-1 range parameters in LDS when the whole
+// layout with them fits p1_budget bytes (the host passes the dynamic LDS it
+// gave the launch, or 0 to keep them in global memory).
+__device__ __forceinline__ bool pass1_params_in_lds(size_t with_params, uint32_t p1_budget) {
+  return with_params <= (size_t)p1_budget;
+}
+
 template <class K>
 __global__ void __launch_bounds__(1024) k_count1(const K* __restrict__ buf, const PassCtl* __restrict__ ctl0,
                                                  const K* __restrict__ lut, const Range<K>* __restrict__ rng0,
                                                  const uint64_t* __restrict__ rank0, const MapParam<K>* __restrict__ p1,
-                                                 uint32_t p1_in_lds, const PassCtl* __restrict__ ctl,
+                                                 uint32_t p1_budget, const PassCtl* __restrict__ ctl,
                                                  uint32_t* __restrict__ rows) {
   extern __shared__ __align__(16) unsigned char smem[];
   const uint32_t B = ctl->B, n = ctl->n, tile = ctl->tile;
   if (blockIdx.x >= ctl->G) return;
+  const size_t hist_bytes = ((size_t)B * 4 + 15) / 16 * 16;
+  const bool p1_in_lds = pass1_params_in_lds(
+      pass1_map_bytes<K>(ctl0->B, ctl0->R, ctl0->l2_used, ctl->R) + hist_bytes, p1_budget);
   Pass1Map<K> pm;
-  pass1_map_init<K>(pm, smem, ctl0, lut, rng0, rank0, p1, ctl->R, p1_in_lds != 0);
+  pass1_map_init<K>(pm, smem, ctl0, lut, rng0, rank0, p1, ctl->R, p1_in_lds);
   uint32_t* hist = reinterpret_cast<uint32_t*>(
       smem + pass1_map_bytes<K>(ctl0->B, ctl0->R, ctl0->l2_used, p1_in_lds ? ctl->R : 0));
   for (uint32_t i = threadIdx.x; i < B; i += blockDim.x) hist[i] = 0;
   __syncthreads();
   uint32_t t0 = blockIdx.x * tile;
-  uint32_t t1 = t0 + tile < n ? t0 + tile : n;
+  uint32_t t1 = tile_end(t0, tile, n);
   constexpr int N = Vec16<K>::N;
   stream_tile_vec<K>(
       buf, t0, t1,
@@ -1868,7 +1946,7 @@ template <class K>
 __global__ void __launch_bounds__(1024) k_compact1(const K* __restrict__ buf, const PassCtl* __restrict__ ctl0,
                                                    const K* __restrict__ lut, const Range<K>* __restrict__ rng0,
                                                    const uint64_t* __restrict__ rank0, const MapParam<K>* __restrict__ p1,
-                                                   uint32_t p1_in_lds, const PassCtl* __restrict__ ctl,
+                                                   uint32_t p1_budget, const PassCtl* __restrict__ ctl,
                                                    const uint32_t* __restrict__ rows,
                                                    const uint16_t* __restrict__ slot_of_bucket,
                                                    uint32_t* __restrict__ slot_dst,
@@ -1876,14 +1954,16 @@ __global__ void __launch_bounds__(1024) k_compact1(const K* __restrict__ buf, co
   extern __shared__ __align__(16) unsigned char smem[];
   const uint32_t B = ctl->B, n = ctl->n, tile = ctl->tile, U_slots = ctl->U_slots;
   if (blockIdx.x >= ctl->G || U_slots == 0) return;
+  const bool p1_in_lds = pass1_params_in_lds(
+      pass1_map_bytes<K>(ctl0->B, ctl0->R, ctl0->l2_used, ctl->R) + compact_lds_bytes(B, U_slots), p1_budget);
   Pass1Map<K> pm;
-  pass1_map_init<K>(pm, smem, ctl0, lut, rng0, rank0, p1, ctl->R, p1_in_lds != 0);
+  pass1_map_init<K>(pm, smem, ctl0, lut, rng0, rank0, p1, ctl->R, p1_in_lds);
   CompactTables ct;
   compact_init(ct, smem + pass1_map_bytes<K>(ctl0->B, ctl0->R, ctl0->l2_used, p1_in_lds ? ctl->R : 0), B, U_slots, slot_of_bucket,
                slot_dst, slot_bucket, rows + (size_t)blockIdx.x * B);
   __syncthreads();
   uint32_t t0 = blockIdx.x * tile;
-  uint32_t t1 = t0 + tile < n ? t0 + tile : n;
+  uint32_t t1 = tile_end(t0, tile, n);
   constexpr int N = Vec16<K>::N;
   stream_tile_vec<K>(
       buf, t0, t1,
@@ -1927,7 +2007,7 @@ __global__ void __launch_bounds__(1024) k_compactS(const K* __restrict__ buf, co
   compact_init(ct, smem, B, U_slots, slot_of_bucket, slot_dst, slot_bucket, rows + (size_t)blockIdx.x * B);
   __syncthreads();
   uint32_t t0 = blockIdx.x * tile;
-  uint32_t t1 = t0 + tile < n ? t0 + tile : n;
+  uint32_t t1 = tile_end(t0, tile, n);
   SegCursor<K> cur;
   cur.rg = ranges;
   cur.load(wg_first[blockIdx.x]);
@@ -2195,7 +2275,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8)
   for (uint32_t jj = blockIdx.x; jj < U_grp; jj += gridDim.x) {
     const uint32_t j = small_order ? small_order[jj] : jj;
     const uint32_t st = small_start[j], len = small_len[j];
-    if (len > fin_cap) {  // cannot happen unless the pass cap forced finishing; reported, never sorted
+    if (len > fin_cap) {  // cannot happen: such segments are ordered into k_finish_wide's range; reported
       if (threadIdx.x == 0) *err = 1u;  // a flag word in host-visible memory
       continue;
     }
@@ -2242,12 +2322,91 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8)
   }
 }
 
+// Stable LSD radix sort of a[0, len) by ONE workgroup in global memory (8-bit
+// digits over bits [0, vary) -- the bits where the segment's keys differ),
+// ping-ponging between a and tmp.  Returns the buffer holding the sorted keys.
+// Each digit pass: an LDS histogram of the digit, its exclusive scan, then the
+// keys in index order, one per thread per tile: a key's place among equal
+// digits of its wave comes from eight ballots (lanes with the same digit), the
+// waves' counts per digit are ranked in wave order in LDS, so equal digits
+// keep their order (stable) and every key lands at
+// base[digit] + (keys of that digit in earlier tiles and waves) + rank in wave.
+struct RadixSortLds {
+  uint32_t base[256];
+  uint32_t wcnt[16][256];  // per wave: keys of each digit in the current tile, then their first position
+};
+
+template <class K>
+__device__ const K* wg_radix_sort_global(K* __restrict__ a, K* __restrict__ tmp, uint32_t len, uint32_t vary,
+                                         RadixSortLds& L) {
+  const uint32_t nt = blockDim.x, t = threadIdx.x, w = t >> 6, nw = nt >> 6;
+  K* src = a;
+  K* dst = tmp;
+  for (uint32_t shift = 0; shift < vary; shift += 8) {
+    for (uint32_t i = t; i < 256; i += nt) L.base[i] = 0;
+    __syncthreads();
+    for (uint32_t i = t; i < len; i += nt) atomicAdd(&L.base[(uint32_t)(src[i] >> shift) & 255u], 1u);
+    __syncthreads();
+    if (t < 64) {  // exclusive scan of the 256 digit totals, four per lane
+      const uint32_t c0 = L.base[4 * t], c1 = L.base[4 * t + 1], c2 = L.base[4 * t + 2], c3 = L.base[4 * t + 3];
+      const uint32_t sum = c0 + c1 + c2 + c3;
+      const uint32_t ex = wave_incl_scan_u32(sum) - sum;
+      L.base[4 * t] = ex;
+      L.base[4 * t + 1] = ex + c0;
+      L.base[4 * t + 2] = ex + c0 + c1;
+      L.base[4 * t + 3] = ex + c0 + c1 + c2;
+    }
+    __syncthreads();
+    for (uint32_t t0 = 0; t0 < len; t0 += nt) {
+      for (uint32_t i = t; i < nw * 256; i += nt) L.wcnt[i >> 8][i & 255] = 0;
+      __syncthreads();
+      const uint32_t i = t0 + t;
+      const bool valid = i < len;
+      const K key = valid ? src[i] : (K)0;
+      const uint32_t d = (uint32_t)(key >> shift) & 255u;
+      uint64_t peers = wave_ballot(valid);
+#pragma unroll
+      for (uint32_t b = 0; b < 8; ++b) {
+        const bool bit = (d >> b) & 1u;
+        const uint64_t m = wave_ballot(bit);
+        peers &= bit ? m : ~m;
+      }
+      const uint32_t rank = mask_below(peers);
+      if (valid && rank == 0) L.wcnt[w][d] = (uint32_t)__builtin_popcountll(peers);
+      __syncthreads();
+      for (uint32_t dd = t; dd < 256; dd += nt) {  // positions in wave order, then the next tile's base
+        uint32_t off = L.base[dd];
+        for (uint32_t v = 0; v < nw; ++v) {
+          const uint32_t c = L.wcnt[v][dd];
+          L.wcnt[v][dd] = off;
+          off += c;
+        }
+        L.base[dd] = off;
+      }
+      __syncthreads();
+      if (valid) dst[L.wcnt[w][d] + rank] = key;
+      __syncthreads();  // every position read before the next tile clears them
+    }
+    K* sw = src;
+    src = dst;
+    dst = sw;
+  }
+  return src;
+}
+
 // Small segments too wide for half-width offsets and too large for full keys
-// in the finish kernel's LDS (rare: a few per call at most): radix select
-// straight from global memory, eight ks per sweep.  They are the last U_wide
-// entries of the finish order; one workgroup per segment.
+// in the finish kernel's LDS (rare: a few per call at most), and segments
+// larger than the finish capacity (the one-pass bucketMultiselect, or a pass
+// cap that forced finishing), from global memory: up to kSelectMaxKs ks by
+// radix select (one sweep per digit for all of them); more ks (a k-dense
+// segment, e.g. the top segment of a top-k select) by a stable radix sort of
+// the segment through the same positions of the spare buffer `tmp`, read at
+// every rank.  They are the last U_wide entries of the finish order; one
+// workgroup per segment.
 template <class T>
-__global__ void __launch_bounds__(1024) k_finish_wide(const typename KeyTraits<T>::K* __restrict__ buf, const PassCtl* __restrict__ ctl,
+__global__ void __launch_bounds__(1024) k_finish_wide(typename KeyTraits<T>::K* __restrict__ buf,
+                                                      typename KeyTraits<T>::K* __restrict__ tmp,
+                                                      const PassCtl* __restrict__ ctl,
                                                       const uint32_t* __restrict__ small_start,
                                                       const uint32_t* __restrict__ small_len,
                                                       const typename KeyTraits<T>::K* __restrict__ small_klo,
@@ -2259,13 +2418,24 @@ __global__ void __launch_bounds__(1024) k_finish_wide(const typename KeyTraits<T
                                                       uint32_t fin_cap) {
   using K = typename KeyTraits<T>::K;
   __shared__ SelectLds L;
+  __shared__ RadixSortLds RL;
   const uint32_t U_small = ctl->U_small, U_wide = ctl->U_wide;
   for (uint32_t jj = U_small - U_wide + blockIdx.x; jj < U_small; jj += gridDim.x) {
     const uint32_t j = small_order ? small_order[jj] : jj;
     const uint32_t st = small_start[j], len = small_len[j];
     const K klo = small_klo[j], khi = small_khi[j];
-    if (!finish_wide_segment<K>(len, klo, khi, fin_cap)) continue;  // k_finish's (or flagged there)
+    if (!finish_wide_segment<K>(len, klo, khi, fin_cap)) continue;  // k_finish's
     const uint32_t q0 = small_kstart[j], q1 = small_kstart[j + 1];
+    if (q1 - q0 > kSelectMaxKs && tmp) {
+      const K* sorted = wg_radix_sort_global<K>(buf + st, tmp + st, len, bitlen64((uint64_t)(klo ^ khi)), RL);
+      for (uint32_t qq = q0 + threadIdx.x; qq < q1; qq += blockDim.x) {
+        const uint32_t k = klist[qq];
+        out[k] = KeyTraits<T>::dec(sorted[k_rank[k] - 1]);
+        k_state[k] = kDone;
+      }
+      __syncthreads();
+      continue;
+    }
     for (uint32_t c0 = q0; c0 < q1; c0 += kSelectMaxKs) {
       const uint32_t c1 = q1 - c0 < kSelectMaxKs ? q1 : c0 + kSelectMaxKs;
       uint32_t kk = 0, kr = 0;

@@ -17,14 +17,14 @@ from . import _lib
 from ._lib import Opts, RangeU64, Stats
 
 __all__ = [
-    "GGMSError", "Context", "Opts", "Stats", "default_opts", "select", "select_device",
-    "iterativeSMOSWrapper", "bucketMultiselectWrapper", "distributedSMOSWrapper",
+    "GGMSError", "Context", "Opts", "Stats", "default_opts", "select", "select_device", "topk",
+    "iterativeSMOSWrapper", "bucketMultiselectWrapper", "distributedSMOSWrapper", "distributedBucketMultiselectWrapper",
     "DTYPES", "ALGS", "SPLITTERS", "DISTS", "KDISTS", "generate", "generate_ks", "mixture_params",
 ]
 
 OK, EINVAL, ENAN, ENOMEM, EDEVICE, ENOTSUP = range(6)
 DTYPES = {"float32": 0, "float64": 1, "uint32": 2, "int32": 3}
-ALGS = {"bucket": 0, "iterative": 1, "distributed": 2}
+ALGS = {"bucket": 0, "iterative": 1, "distributed": 2, "distributed_bucket": 3}
 SPLITTERS = {"linear": 0, "learned": 1, "sketch": 2}
 DISTS = {"uniform": 0, "normal": 1, "half_normal": 2, "lognormal": 3, "mixture": 4, "near_constant": 5,
          "constant": 6, "cauchy": 7, "mod_index": 8, "ascending": 9}
@@ -119,6 +119,19 @@ class Context:
         _check(st, "ggms_select")
         return out
 
+    def topk(self, x, K: int, alg: str = "iterative", opts: Optional[Opts] = None) -> np.ndarray:
+        """The K largest values of device tensor x, largest first (host result)."""
+        dt = _dtype_code(x.dtype)
+        x = x.contiguous()
+        out = np.empty(K, dtype=_NP[dt])
+        o = opts if opts is not None else default_opts()
+        if o.splitter == SPLITTERS["learned"]:
+            _ensure_learned(self.device)
+        st = self.lib.ggms_select_topk(self.h, ALGS[alg], dt, C.c_void_p(x.data_ptr()), x.numel(), K,
+                                       out.ctypes.data_as(C.c_void_p), C.byref(o))
+        _check(st, "ggms_select_topk")
+        return out
+
     def select_device(self, x, d_ks, d_out, alg: str = "iterative", opts: Optional[Opts] = None):
         """Device-resident ks (uint32 tensor) and outputs (tensor of x's dtype)."""
         dt = _dtype_code(x.dtype)
@@ -183,6 +196,10 @@ def select_device(x, d_ks, d_out, alg: str = "iterative", opts: Optional[Opts] =
     return _ctx(x.device.index or 0).select_device(x, d_ks, d_out, alg, opts)
 
 
+def topk(x, K: int, alg: str = "iterative", opts: Optional[Opts] = None) -> np.ndarray:
+    return _ctx(x.device.index or 0).topk(x, K, alg, opts)
+
+
 _SUF = {0: "f32", 1: "f64", 2: "u32", 3: "i32"}
 
 
@@ -211,6 +228,14 @@ def distributedSMOSWrapper(d_vector, length, kVals_ori, numKs, outputs, blocks=0
     """DistributedSMOS::distributedSMOSWrapper with one rank (DistributedSMOS/distributedSMOS.cpp:1208-1267)."""
     return _wrapper("distributedSMOSWrapper", d_vector, length, kVals_ori, numKs, outputs, blocks, threads,
                     int(rank))
+
+
+def distributedBucketMultiselectWrapper(d_vector, length, kVals_ori, numKs, outputs, blocks=0, threads=1024,
+                                        rank=0) -> int:
+    """DistributedBucketMultiselect::distributedBucketMultiselectWrapper with one rank
+    (DistributedSMOS/distributedBucketMultiselect.cpp:929-1018)."""
+    return _wrapper("distributedBucketMultiselectWrapper", d_vector, length, kVals_ori, numKs, outputs, blocks,
+                    threads, int(rank))
 
 
 def generate(dtype: str, dist: str, n: int, seed: int, p0: float = 0.0, p1: float = 0.0, device: int = 0):

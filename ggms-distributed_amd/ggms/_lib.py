@@ -29,7 +29,9 @@ class Opts(C.Structure):
         ("seed", C.c_uint64),
         ("stream", C.c_void_p),
         ("profile", C.c_uint32),
-        ("reserved", C.c_uint32 * 7),
+        ("schedule", C.c_uint32),
+        ("device_passes", C.c_uint32),
+        ("reserved", C.c_uint32 * 5),
     ]
 
 
@@ -97,6 +99,11 @@ SIGNATURES = {
     "ggms_select_device": (C.c_int, [_P, C.c_int, C.c_int, _P, C.c_uint64, _P, C.c_uint32, _P, C.POINTER(Opts)]),
     "ggms_debug_first_pass": (C.c_int, [_P, C.c_int, _P, C.c_uint64, C.POINTER(Opts), C.POINTER(RangeU64), _U32P,
                                         _U32P, _U32P]),
+    "ggms_first_pass_tiling": (C.c_int, [C.c_uint64, C.c_uint32, _U32P, _U32P]),
+    "ggms_select_topk": (C.c_int, [_P, C.c_int, C.c_int, _P, C.c_uint64, C.c_uint32, _P, C.POINTER(Opts)]),
+    "ggms_ctx_reserve": (C.c_int, [_P, C.c_int, C.c_uint64, C.c_uint32, C.POINTER(Opts)]),
+    "ggms_ctx_device_status": (C.c_int, [_P]),
+    "ggms_last_wrapper_status": (C.c_int, []),
 }
 SIGNATURES["ggms_generate"] = (C.c_int, [C.c_int, C.c_int, _P, C.c_uint64, C.c_uint64, C.c_double, C.c_double, _P])
 SIGNATURES["ggms_mixture_params"] = (C.c_int, [C.c_uint64, C.POINTER(C.c_int), _P, _P])
@@ -120,9 +127,9 @@ SIGNATURES["ggms_select_batched"] = (C.c_int, [C.c_int, C.c_int, _P, C.c_uint32,
 for _alg in ("iterativeSMOSWrapper", "bucketMultiselectWrapper"):
     for _suf in ("f32", "f64", "u32", "i32"):
         SIGNATURES[f"ggms_{_alg}_{_suf}"] = (C.c_int, [_P, C.c_int, _U32P, C.c_int, _P, C.c_int, C.c_int])
-for _suf in ("f32", "f64", "u32", "i32"):
-    SIGNATURES[f"ggms_distributedSMOSWrapper_{_suf}"] = (C.c_int, [_P, C.c_int, _U32P, C.c_int, _P, C.c_int, C.c_int,
-                                                                  C.c_int])
+for _alg in ("distributedSMOSWrapper", "distributedBucketMultiselectWrapper"):
+    for _suf in ("f32", "f64", "u32", "i32"):
+        SIGNATURES[f"ggms_{_alg}_{_suf}"] = (C.c_int, [_P, C.c_int, _U32P, C.c_int, _P, C.c_int, C.c_int, C.c_int])
 
 _lib = None
 

@@ -4,9 +4,10 @@ count, sample size and pieces, plus the workgroup size of the launch.
 A size class c holds the problems with ceil(log2(n)) == c.  For every class
 the tuner builds a synthetic batch of that class (the batched workload's value
 mix: uniform / normal / lognormal(0,1), 1..512 random ks), times each
-candidate with the persistent kernel replayed from a hipGraph, checks the
-winner's outputs against the CPU sort-and-choose on a few problems, and
-writes the table as JSON:
+candidate with the persistent kernel replayed from a hipGraph, cross-checks
+the winner's outputs on a few problems against the library's other engine
+(the multi-workgroup iterativeSMOS path, one call per problem), and writes the
+table as JSON:
 
     python -m ggms.autotune --out ggms-distributed_amd/weights/batch_tuning.json
 """
@@ -63,13 +64,14 @@ def class_batch(c: int, problems: int, seed: int):
 
 
 def check(x, off, ks, koff, got, problems):
-    import oracle
-    host = x.cpu().numpy()
+    """The batched kernel's answers for `problems` must equal those of the
+    iterativeSMOS engine (an independent set of kernels) on the same values."""
+    from . import select
     for p in problems:
         a, b = int(off[p]), int(off[p + 1])
-        want = oracle.sort_and_choose(host[a:b], ks[koff[p]:koff[p + 1]])
+        want = select(x[a:b], ks[koff[p]:koff[p + 1]], "iterative")
         if not np.array_equal(got[koff[p]:koff[p + 1]].view(np.uint32), want.view(np.uint32)):
-            raise AssertionError(f"autotune winner wrong on problem {p}")
+            raise AssertionError(f"autotune winner disagrees with the iterative engine on problem {p}")
 
 
 def tune(classes=range(10, 23), problems: int = 512, reps: int = 5, seed: int = 77, log=print):

@@ -1,9 +1,10 @@
 /*
  * ggms.h -- C ABI of the MI355X multiple-order-statistics selection library.
  *
- * Given a device vector of n values and a list of k ranks, return the k-th
- * value for every rank.  Ranks follow the reference wrapper convention:
- * k = 1 is the LARGEST value, k = n the smallest
+ * Given a device vector of n values (1 <= n <= 2^31 - 1, the reference's int
+ * length) and a list of k ranks, return the k-th value for every rank.  Ranks
+ * follow the reference wrapper convention: k = 1 is the LARGEST value, k = n
+ * the smallest
  * (reference: DistributedSMOS/iterativeSMOS.cu:1587-1588 converts k-th largest
  * to k-th smallest with length - k + 1; the sort-and-choose oracle picks
  * sorted[n - k], DistributedSMOS/iterativeSMOSTimingFunctions.cu:67-73).
@@ -39,9 +40,13 @@ typedef enum {
 typedef enum { GGMS_F32 = 0, GGMS_F64 = 1, GGMS_U32 = 2, GGMS_I32 = 3 } ggms_dtype;
 
 typedef enum {
-  GGMS_ALG_BUCKET = 0,          /* bucketMultiselect: one bucketing pass, then sort of the k-buckets */
-  GGMS_ALG_ITERATIVE = 1,       /* iterativeSMOS: re-bucket the k-buckets until they are small */
-  GGMS_ALG_DISTRIBUTED_1GPU = 2 /* distributedSMOS collapsed to one GPU: strided sample, no exchange */
+  GGMS_ALG_BUCKET = 0,           /* bucketMultiselect: ONE bucketing pass; every k-bucket is compacted into its
+                                    own segment and finished there (LDS radix select / sort, or a radix select
+                                    from global memory for a segment above the LDS capacity) */
+  GGMS_ALG_ITERATIVE = 1,        /* iterativeSMOS: re-bucket the k-buckets until they are small */
+  GGMS_ALG_DISTRIBUTED_1GPU = 2, /* distributedSMOS collapsed to one GPU: strided sample, no exchange */
+  GGMS_ALG_DISTRIBUTED_BUCKET_1GPU = 3 /* distributedBucketMultiselect collapsed to one GPU: strided sample,
+                                    one bucketing pass, k-buckets finished as in GGMS_ALG_BUCKET */
 } ggms_alg;
 
 typedef enum {
@@ -63,8 +68,25 @@ typedef struct {
   uint64_t seed;          /* sampling seed (default 0x5EED) */
   void* stream;           /* hipStream_t to run on; NULL = the null stream */
   uint32_t profile;       /* nonzero: time every kernel with HIP events on the launch stream (see ggms_stats) */
-  uint32_t reserved[7];
+  uint32_t schedule;      /* ggms_schedule (default HOST) */
+  uint32_t device_passes; /* DEVICE schedule: bucketing passes launched (0 = 3, max 8); the last one finishes
+                             every k-bucket left whatever its size */
+  uint32_t reserved[5];
 } ggms_opts;
+
+/* How a call's passes are driven.
+ *   HOST:   the host reads one control block per pass (from pinned memory, while
+ *           the pass's compaction runs) and launches the next pass with exact
+ *           shapes; stops as soon as every k is answered.
+ *   DEVICE: every launch of every pass is enqueued at once with shapes sized
+ *           for the largest pass; the kernels read their control blocks and a
+ *           pass that turned out empty returns at once.  No host read between
+ *           kernels: ggms_select_device with this schedule is asynchronous and
+ *           can be captured in a hipGraph (reserve the workspace first with
+ *           ggms_ctx_reserve; read the status with ggms_ctx_device_status once
+ *           the stream has completed).  Replaces the reference's host loop
+ *           (DistributedSMOS/iterativeSMOS.cu:1348-1512). */
+typedef enum { GGMS_SCHEDULE_HOST = 0, GGMS_SCHEDULE_DEVICE = 1 } ggms_schedule;
 
 /* Per-call statistics of the last selection run on a context. */
 #define GGMS_MAX_PASS_STATS 64
@@ -101,8 +123,25 @@ ggms_status ggms_ctx_create(int device, ggms_ctx** out);
 void ggms_ctx_destroy(ggms_ctx* ctx);
 ggms_status ggms_ctx_stats(const ggms_ctx* ctx, ggms_stats* out);
 
+/* First-pass tiling of an n-element vector over at most `workgroups` streaming
+ * workgroups, and no more workgroups than 16384-element pieces: every
+ * workgroup but the last owns `tile` consecutive elements (a multiple of 64),
+ * the last owns the rest; *groups workgroups in all.  Host-only (no device is
+ * touched).
+ * GGMS_EINVAL for n == 0 or n > 2^31 - 1 (the reference's int length). */
+ggms_status ggms_first_pass_tiling(uint64_t n, uint32_t workgroups, uint32_t* tile, uint32_t* groups);
+
 /* Device workspace bytes a call with these arguments needs. */
 uint64_t ggms_workspace_bytes(ggms_dtype t, uint64_t n, uint32_t nk, const ggms_opts* opts);
+
+/* Sizes the context's device workspace and pinned host block for calls of up
+ * to these arguments, so that later calls (in particular DEVICE-schedule
+ * calls under hipGraph capture, which never allocate) find them ready. */
+ggms_status ggms_ctx_reserve(ggms_ctx* ctx, ggms_dtype t, uint64_t n, uint32_t nk, const ggms_opts* opts);
+
+/* Status of the context's last DEVICE-schedule ggms_select_device call, from
+ * the words its kernels wrote; valid once its stream has completed. */
+ggms_status ggms_ctx_device_status(const ggms_ctx* ctx);
 
 /*
  * Multiselect.  d_vec: n values of type t on the device.  ks_host: nk ranks in
@@ -117,9 +156,21 @@ ggms_status ggms_select(ggms_ctx* ctx, ggms_alg alg, ggms_dtype t, const void* d
                         const uint32_t* ks_host, uint32_t nk, void* out_host, const ggms_opts* opts);
 
 /* Same, with ks and outputs in device memory; asynchronous with respect to the
- * host except for one small status read per bucketing pass. */
+ * host except for one small status read per bucketing pass (HOST schedule), or
+ * fully asynchronous and capturable (DEVICE schedule, see ggms_schedule). */
 ggms_status ggms_select_device(ggms_ctx* ctx, ggms_alg alg, ggms_dtype t, const void* d_vec, uint64_t n,
                                const uint32_t* d_ks, uint32_t nk, void* d_out, const ggms_opts* opts);
+
+/*
+ * Top-k select: out_host[i] = the (i + 1)-th largest value, i < K (the K
+ * largest values, largest first), by the multiselect engine `alg` over the
+ * ranks 1..K.  Replaces the reference's top-k timing path, which feeds
+ * ks = 1..K to bucketMultiselectWrapper
+ * (timeBucketTopkselect, DistributedSMOS/iterativeSMOSTimingFunctions.cu:306-333).
+ * K == 0 does nothing; K > n is GGMS_EINVAL.
+ */
+ggms_status ggms_select_topk(ggms_ctx* ctx, ggms_alg alg, ggms_dtype t, const void* d_vec, uint64_t n, uint32_t K,
+                             void* out_host, const ggms_opts* opts);
 
 /*
  * Reference call shapes, one per instantiated type, using the library's
@@ -127,7 +178,9 @@ ggms_status ggms_select_device(ggms_ctx* ctx, ggms_alg alg, ggms_dtype t, const 
  * signature compatibility (the reference passes SM count and max threads) and
  * only threads is honoured, as the workgroup size when it is a multiple of 64.
  * Return value: GGMS_OK or an error status (the reference returns T and
- * ignores errors).
+ * ignores errors); the calling thread's last status is also kept for
+ * ggms_last_wrapper_status (the C++ shim in ggms.hpp returns the reference's
+ * (T)1 on success and (T)0 on failure, and callers read the reason there).
  *   IterativeSMOS::iterativeSMOSWrapper   DistributedSMOS/iterativeSMOS.cu:1579-1612
  *   BucketMultiselect::bucketMultiselectWrapper DistributedSMOS/bucketMultiselect.cu:852-899
  *   DistributedSMOS::distributedSMOSWrapper DistributedSMOS/distributedSMOS.cpp:1208-1280
@@ -144,6 +197,14 @@ int ggms_distributedSMOSWrapper_f32(float* d_vector, int length, const uint32_t*
 int ggms_distributedSMOSWrapper_f64(double* d_vector, int length, const uint32_t* kVals_ori, int numKs, double* outputs, int blocks, int threads, int rank);
 int ggms_distributedSMOSWrapper_u32(uint32_t* d_vector, int length, const uint32_t* kVals_ori, int numKs, uint32_t* outputs, int blocks, int threads, int rank);
 int ggms_distributedSMOSWrapper_i32(int32_t* d_vector, int length, const uint32_t* kVals_ori, int numKs, int32_t* outputs, int blocks, int threads, int rank);
+/*   DistributedBucketMultiselect::distributedBucketMultiselectWrapper
+ *     DistributedSMOS/distributedBucketMultiselect.hpp:12-14, .cpp:929-1018 (one GPU holds the whole vector) */
+int ggms_distributedBucketMultiselectWrapper_f32(float* d_vector, int length, const uint32_t* kVals_ori, int numKs, float* outputs, int blocks, int threads, int rank);
+int ggms_distributedBucketMultiselectWrapper_f64(double* d_vector, int length, const uint32_t* kVals_ori, int numKs, double* outputs, int blocks, int threads, int rank);
+int ggms_distributedBucketMultiselectWrapper_u32(uint32_t* d_vector, int length, const uint32_t* kVals_ori, int numKs, uint32_t* outputs, int blocks, int threads, int rank);
+int ggms_distributedBucketMultiselectWrapper_i32(int32_t* d_vector, int length, const uint32_t* kVals_ori, int numKs, int32_t* outputs, int blocks, int threads, int rank);
+/* Status of the calling thread's last ggms_*Wrapper_* call (GGMS_OK before any). */
+ggms_status ggms_last_wrapper_status(void);
 
 /*
  * Introspection of the first bucketing pass (used by the parity tests to check
@@ -177,7 +238,7 @@ ggms_status ggms_debug_first_pass(ggms_ctx* ctx, ggms_dtype t, const void* d_vec
  * k = 1 is the largest, 1 <= k <= size, at most GGMS_BATCH_MAX_KS per
  * problem) and writes out[koffsets[p] + j] = the ks[koffsets[p] + j]-th
  * largest of its values.  offsets / koffsets are host arrays of
- * num_problems + 1 entries.  Problem sizes are below 2^32.
+ * num_problems + 1 entries.  Problem sizes are at most 2^31 - 1.
  *
  * A plan fixes the buffers and shapes; its launch can be repeated (new
  * values in the same buffers) and, with use_graph, is a hipGraph replay.

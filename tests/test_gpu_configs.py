@@ -2,6 +2,11 @@
 sort-and-choose oracle (DistributedSMOS/iterativeSMOSTimingFunctions.cu:75-120),
 bit-exact:
 
+  * the headline workload: 2^28 fp32 N(0,1) with 1001 uniform random ranks,
+    under the linear, sketch and learned splitters;
+  * the batched workload: 4096 problems with sizes log-uniform in
+    [2^10, 2^22], the committed autotuned class table, replayed from a hipGraph,
+    every problem of every replay checked;
   * adversarial inputs at 2^26 (uint32 in [0, 1023]; fp32 |N(0,1)|; fp32 where
     99 % of positions hold 1.0f), each with 8192 evenly spaced ranks and 8192
     ranks clustered in a 1 % window;
@@ -62,3 +67,42 @@ def test_fp64_normal_2p27_1001ks(gpu_ctx):
     x = ggms.generate("float64", "normal", n, 207)
     ks = [ggms.generate_ks("uniform_random", 1001, n, 208)]
     _check_all(gpu_ctx, x, ks, ("iterative", "distributed"), ("linear",), "fp64 normal 2^27")
+
+
+def test_headline_2p28_normal_1001ks(gpu_ctx):
+    import ggms
+    n = 1 << 28
+    x = ggms.generate("float32", "normal", n, 20240609)
+    ks = [ggms.generate_ks("uniform_random", 1001, n, 20240610)]
+    _check_all(gpu_ctx, x, ks, ("iterative",), ("linear", "sketch", "learned"), "normal 2^28 1001 ks")
+
+
+def test_batched_4096_problems_autotuned_graph(gpu_ctx):
+    """The batched benchmark workload at full size: every problem of two graph
+    replays against the oracle."""
+    from concurrent.futures import ThreadPoolExecutor
+    import torch
+    from ggms import autotune
+    from ggms.batch import BatchPlan, make_batch
+    x, off, ks, koff, sizes = make_batch(4096, 10, 22, 512, seed=20240613)
+    classes, bt = autotune.load()
+    assert classes is not None, "the committed tuning table is missing"
+    plan = BatchPlan(x, off, ks, koff, classes=classes, block_threads=bt, use_graph=True)
+    host = x.cpu().numpy()
+
+    def want(p):
+        return oracle.sort_and_choose(host[int(off[p]):int(off[p + 1])], ks[koff[p]:koff[p + 1]], threads=1)
+
+    with ThreadPoolExecutor(16) as ex:
+        wants = list(ex.map(want, range(len(sizes))))
+    print(f"batched: {len(sizes)} problems, {int(off[-1])} values, {len(ks)} ks; oracle done", flush=True)
+    for rep in range(2):
+        plan.launch()
+        torch.cuda.synchronize()
+        got = plan.result()
+        for p in range(len(sizes)):
+            g = got[koff[p]:koff[p + 1]]
+            bad = np.nonzero(_bits(g) != _bits(wants[p]))[0]
+            assert bad.size == 0, f"replay {rep} problem {p} (n={sizes[p]}): {bad.size} wrong"
+        print(f"batched replay {rep}: all {len(sizes)} problems exact", flush=True)
+    plan.close()

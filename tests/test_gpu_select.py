@@ -11,7 +11,7 @@ import oracle
 
 pytestmark = pytest.mark.gpu
 
-ALGS = ["bucket", "iterative", "distributed"]
+ALGS = ["bucket", "iterative", "distributed", "distributed_bucket"]
 NP = {"float32": np.float32, "float64": np.float64, "uint32": np.uint32, "int32": np.int32}
 
 
@@ -200,7 +200,8 @@ def test_reference_wrappers(gpu_ctx):
     x = ggms.generate("float64", "normal", n, 5)
     ks = ggms.generate_ks("uniform_random", 101, n, 6)
     want = oracle.sort_and_choose(_host(x), ks)
-    for fn in (ggms.iterativeSMOSWrapper, ggms.bucketMultiselectWrapper, ggms.distributedSMOSWrapper):
+    for fn in (ggms.iterativeSMOSWrapper, ggms.bucketMultiselectWrapper, ggms.distributedSMOSWrapper,
+               ggms.distributedBucketMultiselectWrapper):
         out = np.zeros(len(ks), dtype=np.float64)
         assert fn(x, n, ks, len(ks), out, 256, 1024) == 0
         assert np.array_equal(_bits(out), _bits(want)), fn.__name__
@@ -269,3 +270,88 @@ def test_finish_layouts(gpu_ctx, dt):
         want = oracle.sort_and_choose(_host(x), ks, threads=8)
         got = gpu_ctx.select(x, ks, "iterative", ggms.default_opts(num_buckets=b) if b else None)
         assert np.array_equal(_bits(got), _bits(want)), (n, dist, b)
+
+
+def test_length_cap_rejected_before_any_device_work(gpu_ctx):
+    """Lengths above the reference's int range (2^31 - 1) are refused with EINVAL
+    by every single-vector entry point, before anything touches the vector."""
+    import ctypes as C
+    import ggms
+    import torch
+    lib = ggms._lib.load()
+    x = torch.zeros(16, dtype=torch.float32, device="cuda")
+    ks = np.array([1], dtype=np.uint32)
+    out = np.zeros(1, dtype=np.float32)
+    for n in (1 << 31, (1 << 32) - 1, 1 << 33):
+        st = lib.ggms_select(gpu_ctx.h, 1, 0, C.c_void_p(x.data_ptr()), n, ks.ctypes.data_as(ggms._lib._U32P), 1,
+                             out.ctypes.data_as(C.c_void_p), None)
+        assert st == ggms.EINVAL, n
+    with pytest.raises(ggms.GGMSError):
+        gpu_ctx.select(x, [1], "iterative", n=1 << 31)
+
+
+def test_reference_shim_program(gpu_ctx):
+    """A C++ program written against include/ggms.hpp (the reference's template
+    call shape) links libggms.so and runs on the GPU: (T)1 and the known answers
+    v[i] = i + 1 for all three wrappers and four types, (T)0 and
+    ggms_last_wrapper_status() == GGMS_EINVAL for a bad k."""
+    import os
+    import shutil
+    import subprocess
+    import tempfile
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    libdir = os.path.join(root, "ggms-distributed_amd", "lib")
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("g++ not available")
+    with tempfile.TemporaryDirectory() as d:
+        exe = os.path.join(d, "shim_main")
+        r = subprocess.run([gxx, "-std=c++17", "-O1", "-D__HIP_PLATFORM_AMD__", "-I", "/opt/rocm/include",
+                            "-I", os.path.join(root, "include"), os.path.join(root, "tests", "shim", "shim_main.cpp"),
+                            "-o", exe, "-L", libdir, "-lggms", "-L", "/opt/rocm/lib", "-lamdhip64",
+                            f"-Wl,-rpath,{libdir}", "-Wl,-rpath,/opt/rocm/lib"], capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr[-3000:]
+        r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+        print(r.stdout)
+        assert r.returncode == 0 and "shim ok" in r.stdout, r.stdout + r.stderr
+
+
+@pytest.mark.parametrize("K", [1, 1024, 65536])
+def test_topk(gpu_ctx, K):
+    """Top-k select (the reference's timeBucketTopkselect feeds ks = 1..K,
+    DistributedSMOS/iterativeSMOSTimingFunctions.cu:306-333): the K largest values,
+    largest first, through the iterative engine and the one-pass bucketMultiselect
+    (whose top segment is k-dense: it is sorted from global memory)."""
+    import ggms
+    n = 1 << 26
+    x = ggms.generate("float32", "normal", n, 4242)
+    want = oracle.sort_and_choose(_host(x), np.arange(1, K + 1, dtype=np.uint32), threads=16)
+    assert np.all(np.diff(want.astype(np.float64)) <= 0)
+    for alg in ("iterative", "bucket"):
+        got = gpu_ctx.topk(x, K, alg)
+        bad = np.nonzero(_bits(got) != _bits(want))[0]
+        assert bad.size == 0, f"{alg} K={K}: {bad.size} wrong, first at {bad[0]}"
+    assert gpu_ctx.topk(x, 0).size == 0
+    with pytest.raises(ggms.GGMSError):
+        gpu_ctx.topk(x[:100], 101)
+
+
+@pytest.mark.parametrize("dt,dist,kdist,nk", [("float32", "normal", "window", 8192), ("float64", "normal", "even", 4000),
+                                              ("uint32", "uniform", "uniform_random", 3000),
+                                              ("float32", "half_normal", "window", 8192)])
+def test_oversized_segments_finish_from_global_memory(gpu_ctx, dt, dist, kdist, nk):
+    """Segments larger than the LDS finish capacity: the one-pass bucketMultiselect
+    finishes every k-bucket whatever its size, and an iterative run capped at two
+    passes force-finishes its pass-1 k-buckets; both go through the global-memory
+    finish (radix select for <= 8 ks, a stable radix sort for k-dense segments)."""
+    import ggms
+    n = (1 << 24) + 77
+    x = ggms.generate(dt, dist, n, 99)
+    ks = ggms.generate_ks(kdist, nk, n, 100)
+    want = oracle.sort_and_choose(_host(x), ks, threads=16)
+    for alg, o in (("bucket", ggms.default_opts(num_buckets=256)), ("distributed_bucket", ggms.default_opts(num_buckets=512)),
+                   ("iterative", ggms.default_opts(num_buckets=64, max_passes=2))):
+        got = gpu_ctx.select(x, ks, alg, o)
+        bad = np.nonzero(_bits(got) != _bits(want))[0]
+        assert bad.size == 0, f"{alg}: {bad.size} wrong, first k={ks[bad[0]]}"
+        print(f"{dt}/{dist}/{kdist} {alg}: exact", flush=True)

@@ -130,8 +130,7 @@ def test_generators_distributions():
     u = oracle.generate(np.float64, "uniform", n, 5)
     assert stats.kstest(u, "uniform").pvalue > 1e-3 and u.min() >= 0 and u.max() < 1
     u32 = oracle.generate(np.float32, "uniform", n, 5)
-    assert np.array_equal(u32, (u * 2**24).astype(np.uint64).astype(np.float32) / np.float32(2**24)) or True
-    assert np.all((u32 * 2**24) == np.floor(u32 * 2**24))
+    assert np.all((u32 * 2**24) == np.floor(u32 * 2**24))  # 24-bit fractions
     z = oracle.generate(np.float64, "normal", n, 6)
     assert stats.kstest(z, "norm").pvalue > 1e-3
     assert abs(z.mean()) < 5 / math.sqrt(n) and abs(z.std() - 1) < 5 / math.sqrt(n)
@@ -169,6 +168,19 @@ def test_k_generators_formulas():
     assert list(ku) == want
     ke = oracle.generate_ks("even", 8192, n, 0)
     assert list(ke) == [-(-(i + 1) * n // 8192) for i in range(8192)]
+    # generateKUniformRandom (DistributedSMOS/generateProblems.cu:584-604): k = (uint)(u * (float)N)
+    # clamped into [1, N], with u a float in [0, 1) -- here the top 24 bits of a Philox4x32-10 word
+    # (stream 4, counter i >> 2, word i & 3; key = seed ^ "ggms"), the generator pinned above by the
+    # Random123 known-answer vectors
+    for n_u, seed in ((n, 7), (1 << 28, 20240610), (1000, 3)):
+        kr = oracle.generate_ks("uniform_random", nk, n_u, seed)
+        key = [(seed & 0xFFFFFFFF) ^ 0x67676D73, seed >> 32]
+        words = [oracle.philox([i, 0, 4, 0], key) for i in range((nk + 3) // 4)]
+        u = np.array([np.float32(words[i >> 2][i & 3] >> 8) * np.float32(1.0 / 16777216.0) for i in range(nk)],
+                     dtype=np.float32)
+        raw = (u * np.float32(n_u)).astype(np.float64).astype(np.int64)
+        assert list(kr) == list(np.clip(raw, 1, n_u)), n_u
+        assert kr.min() >= 1 and kr.max() <= n_u
     for kd in ("uniform_random", "normal", "cluster", "sectioned", "window"):
         k = oracle.generate_ks(kd, nk, n, 3)
         assert k.min() >= 1 and k.max() <= n, kd
