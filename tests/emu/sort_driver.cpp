@@ -2,19 +2,6 @@
 #include <algorithm>
 #include <random>
 #include "select_kernels.hpp"
-thread_local dim3 threadIdx;
-dim3 blockIdx, blockDim, gridDim;
-EmuBarrier g_bar;
-unsigned char* g_dyn_smem;
-struct WaveSync { std::mutex m; std::condition_variable cv; uint32_t arrived = 0, gen = 0; uint32_t v[64]; uint32_t out[64]; };
-static WaveSync g_w[16];
-uint32_t emu_wave_incl_scan(uint32_t v) {
-  uint32_t w = threadIdx.x >> 6, l = threadIdx.x & 63; WaveSync& s = g_w[w];
-  std::unique_lock<std::mutex> lk(s.m); uint32_t g = s.gen; s.v[l] = v;
-  if (++s.arrived == 64) { uint32_t acc = 0; for (int i = 0; i < 64; ++i) { acc += s.v[i]; s.out[i] = acc; } s.arrived = 0; s.gen++; s.cv.notify_all(); }
-  else s.cv.wait(lk, [&] { return s.gen != g; });
-  return s.out[l];
-}
 template <class K, bool SW> int run1(uint32_t M, uint32_t nt, uint64_t seed) {
   std::mt19937_64 rng(seed);
   // SW (swizzled layout between the register phases): 64 canaries behind the M keys must survive

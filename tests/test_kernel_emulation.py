@@ -19,14 +19,40 @@ CSRC = os.path.join(ROOT, "ggms-distributed_amd", "csrc")
 EMU = os.path.join(ROOT, "tests", "emu")
 
 
+_SHARED = re.compile(r"__shared__\s+(?:__align__\(\d+\)\s+)?(?P<type>[A-Za-z_][\w:<>]*)\s+(?P<decls>[^;]+);")
+
+
+def _per_workgroup_shared(src):
+    """Every `__shared__ T a[n], b;` becomes per-workgroup slots:
+    `auto& a = emu_shared<T[n], ID>(); auto& b = emu_shared<T, ID+1>();`."""
+    counter = [0]
+
+    def repl(m):
+        out = []
+        for d in m.group("decls").split(","):
+            d = d.strip()
+            name = re.match(r"[A-Za-z_]\w*", d).group(0)
+            dims = d[len(name):].strip()
+            counter[0] += 1
+            out.append(f"auto& {name} = emu_shared<{m.group('type')}{dims}, {counter[0]}>();")
+        return " ".join(out)
+
+    return _SHARED.sub(repl, src)
+
+
 def _prepare(dst):
     """Copy the kernel headers with the substitutions the emulation needs: the
-    dynamic-LDS declaration becomes a pointer the driver sets, the wave-wide
-    inclusive scan calls the emulated 64-thread scan, and the nontemporal
-    streaming load (a clang vector builtin) becomes a plain load."""
+    dynamic-LDS declaration becomes a pointer the driver sets, every other
+    __shared__ variable a per-workgroup slot, the wave-wide inclusive scan
+    calls the emulated 64-thread scan, and the nontemporal streaming load (a
+    clang vector builtin) becomes a plain load."""
+    base = {"select_kernels.hpp": 1000, "splitter.hpp": 2000, "batch": 3000}
     for f in ["ggms_device.hpp", "philox.hpp", "ggms_internal.hpp", "select_kernels.hpp", "splitter.hpp"]:
         s = open(os.path.join(CSRC, f)).read()
         s = s.replace("extern __shared__ __align__(16) unsigned char smem[];", "unsigned char* smem = g_dyn_smem;")
+        s = _per_workgroup_shared(s) if f in base else s
+        if f in base:
+            s = re.sub(r"emu_shared<(.*?), (\d+)>\(\)", lambda m: f"emu_shared<{m.group(1)}, {base[f] + int(m.group(2))}>()", s)
         if f == "select_kernels.hpp":
             s, n = re.subn(r"__device__ __forceinline__ uint32_t wave_incl_scan_u32\(uint32_t v\) \{.*?\n\}\n",
                            "__device__ __forceinline__ uint32_t wave_incl_scan_u32(uint32_t v) "
@@ -39,6 +65,8 @@ def _prepare(dst):
     s = open(os.path.join(CSRC, "batch.hip")).read()
     s = s[: s.index("using namespace ggms;")]
     s = s.replace("extern __shared__ __align__(16) unsigned char smem[];", "unsigned char* smem = g_dyn_smem;")
+    s = _per_workgroup_shared(s)
+    s = re.sub(r"emu_shared<(.*?), (\d+)>\(\)", lambda m: f"emu_shared<{m.group(1)}, {base['batch'] + int(m.group(2))}>()", s)
     open(os.path.join(dst, "batch_kernel.hpp"), "w").write(s)
 
 
