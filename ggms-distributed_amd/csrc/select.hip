@@ -327,21 +327,28 @@ constexpr size_t kLdsOneWgPerCu = 81 * 1024;
 // needs more than the default 64 KB; each (device, kernel) pair is raised once
 // to what it has needed so far, so repeated calls (and graph capture) make no
 // runtime call.
+struct LdsAttr {
+  int dev;
+  const void* fn;
+  size_t bytes;
+};
 static void set_lds_attr(const void* fn, size_t bytes) {
   if (bytes <= 65536) return;
   static std::mutex mu;
-  static std::vector<std::pair<std::pair<int, const void*>, size_t>> done;
+  static LdsAttr done[512];  // fixed table: no heap use on this path
+  static uint32_t used = 0;
   int dev = 0;
   (void)hipGetDevice(&dev);
   std::lock_guard<std::mutex> lk(mu);
-  for (auto& e : done)
-    if (e.first.first == dev && e.first.second == fn) {
-      if (e.second >= bytes) return;
-      if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) == hipSuccess) e.second = bytes;
+  for (uint32_t i = 0; i < used; ++i)
+    if (done[i].dev == dev && done[i].fn == fn) {
+      if (done[i].bytes >= bytes) return;
+      if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) == hipSuccess)
+        done[i].bytes = bytes;
       return;
     }
-  if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) == hipSuccess)
-    done.push_back({{dev, fn}, bytes});
+  if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) == hipSuccess && used < 512)
+    done[used++] = {dev, fn, bytes};
 }
 
 #define LDS_CHECK(bytes)                              \
@@ -1083,6 +1090,69 @@ static ggms_status reserve_t(ggms_ctx* ctx, uint64_t n, uint32_t nk, const ggms_
   return ensure_pinned(ctx, 6 * sizeof(PassCtl) + 256 + ks_bytes + (size_t)nk * sizeof(T) + 4096);
 }
 
+// ---------------------------------------------------------------------------
+// Scratch of the allocation-free entry point (ggms_select_noalloc): one fixed
+// array per device, whose device address is looked up once per device with
+// hipGetSymbolAddress (in host code the array's name is not a device address).
+// Every ggms_select_noalloc call on a device uses that device's array, so calls
+// that overlap on one device race; the array keeps the last call's contents,
+// and every call rebuilds what it reads (status words cleared by a fill
+// kernel, control blocks by the setup step).
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocScratchBytes = (size_t)GGMS_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static char* g_scratch_ptr[kMaxDevices];
+
+static ggms_status noalloc_scratch(int device, char** out) {
+  if (device < 0 || device >= kMaxDevices) return GGMS_EDEVICE;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+      (void)hipGetLastError();
+      return GGMS_EDEVICE;
+    }
+    g_scratch_ptr[device] = (char*)p;
+  }
+  *out = g_scratch_ptr[device];
+  return GGMS_OK;
+}
+
+template <class T>
+static ggms_status run_noalloc(ggms_alg alg, const T* d_vec, uint64_t n64, const uint32_t* d_ks, uint32_t nk, T* d_out,
+                               const ggms_opts& o) {
+  using K = typename KeyTraits<T>::K;
+  if (n64 == 0 || n64 > kMaxN) return GGMS_EINVAL;
+  if (nk == 0) return GGMS_OK;
+  if (!d_vec || !d_ks || !d_out) return GGMS_EINVAL;
+  int device = 0, cus = 0;
+  if (hipGetDevice(&device) != hipSuccess) return GGMS_EDEVICE;
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+  if (cus <= 0) cus = 256;
+  ggms_opts od = o;
+  od.schedule = GGMS_SCHEDULE_DEVICE;
+  const Config c = resolve_config<K>(alg, n64, nk, od, cus);
+  if (c.mode == kModeLearned && !learned_weights_device(device)) return GGMS_ENOTSUP;
+  const Layout L = make_layout<T>(n64, nk, c);
+  // the status words sit behind the workspace
+  if (L.total + 256 > kNoAllocScratchBytes) return GGMS_ENOMEM;
+  char* base = nullptr;
+  ggms_status s = noalloc_scratch(device, &base);
+  if (s) return s;
+  uint32_t* flags = (uint32_t*)(base + L.total);
+  hipStream_t st = (hipStream_t)o.stream;
+  Prof prof(nullptr, st, false);
+  s = launch_fixed<T>(c, L, base, flags, d_ks, d_out, device, cus, d_vec, (uint32_t)n64, nk, st, prof);
+  if (s) return s;
+  HIP_TRY(hipStreamSynchronize(st));
+  uint32_t words[3] = {0, 0, 0};
+  HIP_TRY(hipMemcpy(words, flags, sizeof(words), hipMemcpyDeviceToHost));
+  return fixed_status(words);
+}
+
 }  // namespace ggms
 
 // ===========================================================================
@@ -1231,6 +1301,20 @@ static ggms_status select_common(ggms_ctx* ctx, ggms_alg alg, ggms_dtype t, cons
 ggms_status ggms_select(ggms_ctx* ctx, ggms_alg alg, ggms_dtype t, const void* d_vec, uint64_t n,
                         const uint32_t* ks_host, uint32_t nk, void* out_host, const ggms_opts* opts) {
   return select_common(ctx, alg, t, d_vec, n, ks_host, false, nk, out_host, false, opts);
+}
+
+ggms_status ggms_select_noalloc(ggms_alg alg, ggms_dtype t, const void* d_vec, uint64_t n, const uint32_t* d_ks,
+                                uint32_t nk, void* d_out, const ggms_opts* opts) {
+  if ((unsigned)alg > 3u) return GGMS_EINVAL;
+  ggms_opts o;
+  if (opts) o = *opts; else ggms_opts_default(&o);
+  switch (t) {
+    case GGMS_F32: return run_noalloc<float>(alg, (const float*)d_vec, n, d_ks, nk, (float*)d_out, o);
+    case GGMS_F64: return run_noalloc<double>(alg, (const double*)d_vec, n, d_ks, nk, (double*)d_out, o);
+    case GGMS_U32: return run_noalloc<uint32_t>(alg, (const uint32_t*)d_vec, n, d_ks, nk, (uint32_t*)d_out, o);
+    case GGMS_I32: return run_noalloc<int32_t>(alg, (const int32_t*)d_vec, n, d_ks, nk, (int32_t*)d_out, o);
+  }
+  return GGMS_EINVAL;
 }
 
 ggms_status ggms_select_device(ggms_ctx* ctx, ggms_alg alg, ggms_dtype t, const void* d_vec, uint64_t n,

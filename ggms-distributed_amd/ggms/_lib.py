@@ -103,6 +103,7 @@ SIGNATURES = {
     "ggms_select_topk": (C.c_int, [_P, C.c_int, C.c_int, _P, C.c_uint64, C.c_uint32, _P, C.POINTER(Opts)]),
     "ggms_ctx_reserve": (C.c_int, [_P, C.c_int, C.c_uint64, C.c_uint32, C.POINTER(Opts)]),
     "ggms_ctx_device_status": (C.c_int, [_P]),
+    "ggms_select_noalloc": (C.c_int, [C.c_int, C.c_int, _P, C.c_uint64, _P, C.c_uint32, _P, C.POINTER(Opts)]),
     "ggms_last_wrapper_status": (C.c_int, []),
 }
 SIGNATURES["ggms_generate"] = (C.c_int, [C.c_int, C.c_int, _P, C.c_uint64, C.c_uint64, C.c_double, C.c_double, _P])

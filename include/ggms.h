@@ -162,6 +162,26 @@ ggms_status ggms_select_device(ggms_ctx* ctx, ggms_alg alg, ggms_dtype t, const 
                                const uint32_t* d_ks, uint32_t nk, void* d_out, const ggms_opts* opts);
 
 /*
+ * Allocation-free multiselect, for callers that cannot allocate device or
+ * pinned memory (code under hipGraph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  No counterpart in the reference.  Same meaning as
+ * ggms_select_device (ks and outputs in device memory, k = 1 the largest),
+ * run with the DEVICE schedule on the current device, on a fixed scratch
+ * array of GGMS_NOALLOC_SCRATCH_BYTES that the library holds on every device
+ * it runs on until the process ends.  Calls no allocating routine; waits for
+ * opts->stream and returns the call's status.  Limit: the call's workspace
+ * (ggms_workspace_bytes with opts->schedule = GGMS_SCHEDULE_DEVICE) plus 256
+ * bytes must fit the scratch -- about 4.2 million values of any type
+ * (2^22 fp32 / u32 / i32 / fp64 with up to tens of thousands of ks); larger
+ * calls return GGMS_ENOMEM without running.  Every call on a device shares
+ * that device's scratch: calls that overlap on one device race (use one host
+ * thread and one stream per device, or serialise).
+ */
+#define GGMS_NOALLOC_SCRATCH_BYTES (96u << 20)
+ggms_status ggms_select_noalloc(ggms_alg alg, ggms_dtype t, const void* d_vec, uint64_t n, const uint32_t* d_ks,
+                                uint32_t nk, void* d_out, const ggms_opts* opts);
+
+/*
  * Top-k select: out_host[i] = the (i + 1)-th largest value, i < K (the K
  * largest values, largest first), by the multiselect engine `alg` over the
  * ranks 1..K.  Replaces the reference's top-k timing path, which feeds
