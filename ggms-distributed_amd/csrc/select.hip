@@ -66,6 +66,7 @@ struct Config {
   uint64_t seed;
   uint32_t force_finish;
   bool one_pass;       // bucketMultiselect: one bucketing pass, every k-bucket finished
+  bool seg_first;      // first pass compacts every k-bucket into its own segment
   bool device_schedule;  // every launch enqueued up front, no host reads between kernels
 };
 
@@ -81,6 +82,19 @@ static Config resolve_config(ggms_alg alg, uint64_t n, uint32_t nk, const ggms_o
   c.mode = kModeLinear;
   if (alg == GGMS_ALG_DISTRIBUTED_1GPU || alg == GGMS_ALG_DISTRIBUTED_BUCKET_1GPU) c.mode = kModeStrided;
   c.one_pass = alg == GGMS_ALG_BUCKET || alg == GGMS_ALG_DISTRIBUTED_BUCKET_1GPU;
+  // The iterative forms carry every first-pass k-bucket into one merged second
+  // pass, which re-buckets them all under one bucket budget: right for few ks,
+  // where the k-buckets are few and large.  With many ks (automatic: at least
+  // one k per five first-pass buckets) the k-buckets are many and most of them
+  // fit the finish, so the first pass writes each into its own segment, those
+  // within the finish capacity are finished at once and only the larger ones
+  // are re-bucketed, each with its own share of the budget.  Measured at 2^28
+  // fp32 N(0,1), merged / segmented: 2048 ks 0.94 / 1.11 ms, 3072 ks
+  // 1.48 / 1.23, 8192 ks (mixture) 3.40 / 2.33.
+  const uint32_t B_auto = o.num_buckets ? o.num_buckets : 14336u;
+  if (c.one_pass || o.first_pass == GGMS_FIRST_PASS_SEGMENTED) c.seg_first = true;
+  else if (o.first_pass == GGMS_FIRST_PASS_MERGED) c.seg_first = false;
+  else c.seg_first = (uint64_t)nk * 5 >= B_auto;
   if (o.splitter == GGMS_SPLIT_LEARNED) c.mode = kModeLearned;
   if (o.splitter == GGMS_SPLIT_SKETCH) c.mode = kModeSketch;
   // 0 = automatic: 14336 for every splitter (the reference uses 8192).  More
@@ -126,7 +140,6 @@ static Config resolve_config(ggms_alg alg, uint64_t n, uint32_t nk, const ggms_o
   c.seed = o.seed;
   c.force_finish = 0;
   (void)n;
-  (void)nk;
   return c;
 }
 
@@ -244,7 +257,8 @@ struct Prof {
 enum Buf {
   bBufA, bBufB, bRows, bTotals, bRanges0, bRanges1, bRangesFirst, bCtl, bKRank, bKState, bKAux, bKList,
   bSlotOfBucket, bSlotDst, bSlotBucket, bActBucket, bActCls, bActDst, bActSlot, bActKlo, bActKhi,
-  bSmallStart, bSmallLen, bSmallKStart, bWgFirst, bLut, bSmallKlo, bSmallKhi, bParams1, bActCnt, bRank0, bSmallOrder, bCount
+  bSmallStart, bSmallLen, bSmallKStart, bWgFirst, bLut, bSmallKlo, bSmallKhi, bParams1, bActCnt, bRank0, bSmallOrder,
+  bGrpCursor, bGrpStart, bGrpIds, bCount
 };
 static_assert(bCount <= kMaxBuffers, "workspace layout table too small");
 
@@ -255,7 +269,7 @@ static Layout make_layout(uint64_t n, uint32_t nk, const Config& c) {
   size_t Bm = std::max<size_t>(c.B0_bound, c.B_cap) + 64;
   size_t Rm = std::max<size_t>(c.R0_bound, Bm) + 64;
   size_t nkp = std::max<uint32_t>(nk, 1);
-  size_t nbuf = n <= c.fin_cap ? n : n;
+  const size_t nbuf = n;
   sz[bBufA] = nbuf * sizeof(K);
   sz[bBufB] = nbuf * sizeof(K);  // also the spare buffer of the finish's global-memory sort
   // one histogram row per workgroup: the first pass's tiles, or one per CU after it
@@ -291,6 +305,10 @@ static Layout make_layout(uint64_t n, uint32_t nk, const Config& c) {
   sz[bParams1] = Bm * sizeof(MapParam<K>);
   sz[bSmallKlo] = Bm * sizeof(K);
   sz[bSmallKhi] = Bm * sizeof(K);
+  sz[bGrpCursor] = (size_t)kMaxGroups * 4;
+  sz[bGrpStart] = (size_t)(kMaxGroups + 1) * 4;
+  // slot index inside its group of every key of the grouped compaction's level 1
+  sz[bGrpIds] = c.seg_first ? nbuf : 0;
   Layout L{};
   size_t o = 0;
   for (int i = 0; i < bCount; ++i) {
@@ -313,6 +331,11 @@ static size_t count_lds0_total(const Config& c) {
   using K = typename KeyTraits<T>::K;
   return count_lds0<T>(c) + LutGeom<K>::bytes();
 }
+
+// The one-pass bucketMultiselect finishes after its bucketing pass every
+// k-bucket of up to this many times the LDS capacity (the larger ones from
+// global memory); only k-buckets above that are re-bucketed first.
+constexpr uint32_t kOnePassFinishCap = 4;
 
 // at most this many ks: the setup runs inside the first-pass planning kernel
 constexpr uint32_t kSetupInPlan0MaxKs = kSetupKsPerThread * 1024;
@@ -410,6 +433,7 @@ static PlanArgs<K> plan_args_common(const Config& c, const Layout& L, char* W, u
   pl.klist = (uint32_t*)P(bKList);
   pl.out = d_out;
   pl.fin_cap = c.fin_cap;
+  pl.finish_cap = c.fin_cap;
   pl.B_req = c.B;
   pl.B_cap = c.B_cap;
   pl.G_max = c.G_max_next;
@@ -512,7 +536,7 @@ static ggms_status launch_fixed(const Config& c, const Layout& L, char* W, uint3
   const uint32_t NT = c.NT;
   const uint32_t Gn = c.G_max_next;
   const uint32_t U_slot_bound = std::min<uint32_t>(nk, std::max(c.B0_bound, c.B_cap));
-  const uint32_t passes = c.one_pass ? 1u : c.max_passes;
+  const uint32_t passes = c.max_passes;
   for (uint32_t pass = 0; pass < passes; ++pass) {
     PassCtl* cp = ctl + pass;
     Range<K>* rc = pass == 0 ? rng_first : rng[pass & 1];
@@ -532,7 +556,7 @@ static ggms_status launch_fixed(const Config& c, const Layout& L, char* W, uint3
       prof.begin(GGMS_KERNEL_COUNT0, (int)pass);
       k_count0<T><<<(G0 + tpw - 1) / tpw, NT, lds, st>>>(d_vec, rc, (K*)P(bLut), cp, rows, tpw);
       prof.end();
-    } else if (pass == 1) {
+    } else if (pass == 1 && !c.seg_first) {
       const size_t lds = pass1_lds_fixed<K>(c, ((size_t)B_bound * 4 + 15) / 16 * 16);
       LDS_CHECK(lds);
       set_lds_attr((const void*)k_count1<K>, lds);
@@ -559,12 +583,14 @@ static ggms_status launch_fixed(const Config& c, const Layout& L, char* W, uint3
     pl.next_ranges = rng[(pass + 1) & 1];
     pl.slot_of_bucket = slot_tab[pass & 1];
     pl.wg_first_next = (uint32_t*)P(bWgFirst) + (size_t)((pass + 1) & 1) * c.G_max;
-    pl.force_finish = (c.one_pass || pass + 1 >= passes) ? 1u : 0u;
-    pl.merge_small = (pass == 0 && !c.one_pass) ? 1u : 0u;
-    pl.next_params = (pass == 0 && !c.one_pass) ? (MapParam<K>*)P(bParams1) : nullptr;
-    pl.rank_words = (pass == 0 && !c.one_pass) ? (uint64_t*)P(bRank0) : nullptr;
+    pl.force_finish = (pass + 1 >= passes) ? 1u : 0u;
+    pl.finish_cap = (pass == 0 && c.one_pass) ? kOnePassFinishCap * c.fin_cap : c.fin_cap;
+    pl.merge_small = (pass == 0 && !c.seg_first) ? 1u : 0u;
+    pl.next_params = (pass == 0 && !c.seg_first) ? (MapParam<K>*)P(bParams1) : nullptr;
+    pl.rank_words = (pass == 0 && !c.seg_first) ? (uint64_t*)P(bRank0) : nullptr;
     pl.mirror = nullptr;
     pl.nan_out = pass == 0 ? flags + 2 : nullptr;
+    plan_groups<K>(pl, L, W, pass == 0 && c.seg_first);
     const size_t lds_plan = (size_t)B_bound * 8 + (size_t)rank_word_count(B_bound) * 8;
     LDS_CHECK(lds_plan);
     set_lds_attr((const void*)k_plan<T>, lds_plan);
@@ -573,14 +599,9 @@ static ggms_status launch_fixed(const Config& c, const Layout& L, char* W, uint3
     prof.end();
     HIP_TRY(hipGetLastError());
     // compaction
-    if (pass == 0 && c.one_pass) {
-      const size_t lds = Pass0Map<K>::param_bytes(c.R0_bound) + compact_lds_bytes(B_bound, U_slot_bound + 4);
-      LDS_CHECK(lds + LutGeom<K>::bytes());
-      set_lds_attr((const void*)k_compact0_seg<T>, lds);
-      prof.begin(GGMS_KERNEL_COMPACT0, (int)pass);
-      k_compact0_seg<T><<<G0, NT, lds, st>>>(d_vec, rc, (K*)P(bLut), cp, rows, pl.slot_of_bucket, pl.slot_dst,
-                                             pl.slot_bucket, dst);
-      prof.end();
+    if (pass == 0 && c.seg_first) {
+      ggms_status s1 = launch_segmented_first_pass<T>(c, L, W, d_vec, n, G0, cp, pl, src, dst, st, prof);
+      if (s1) return s1;
     } else if (pass == 0) {
       const size_t lds = (size_t)c.R0_bound * sizeof(MapParam<K>) + (size_t)B_bound * 2;
       LDS_CHECK(lds + LutGeom<K>::bytes());
@@ -588,7 +609,7 @@ static ggms_status launch_fixed(const Config& c, const Layout& L, char* W, uint3
       prof.begin(GGMS_KERNEL_COMPACT0, (int)pass);
       k_compact0<T><<<G0, NT, lds, st>>>(d_vec, rc, (K*)P(bLut), cp, rows, pl.slot_of_bucket, pl.slot_bucket, dst);
       prof.end();
-    } else if (pass == 1) {
+    } else if (pass == 1 && !c.seg_first) {
       const size_t lds = pass1_lds_fixed<K>(c, compact_lds_bytes(B_bound, U_slot_bound + 4));
       LDS_CHECK(lds);
       set_lds_attr((const void*)k_compact1<K>, lds);
@@ -608,7 +629,7 @@ static ggms_status launch_fixed(const Config& c, const Layout& L, char* W, uint3
     }
     HIP_TRY(hipGetLastError());
     // finish (the iterative first pass hands every k-bucket to pass 1)
-    if (pass > 0 || c.one_pass) {
+    if (pass > 0 || c.seg_first) {
       prof.begin(GGMS_KERNEL_FINISH, (int)pass);
       k_finish<T><<<fin_grid, 1024, fin_lds, st>>>(dst, cp, pl.small_start, pl.small_len, pl.small_klo,
                                                     pl.small_khi, pl.small_kstart, pl.small_order, klist, k_rank,
@@ -621,6 +642,49 @@ static ggms_status launch_fixed(const Config& c, const Layout& L, char* W, uint3
     }
   }
   return GGMS_OK;
+}
+
+// Segmented first pass in two levels (k_group_pass0, then k_group_split):
+// level 1 moves every survivor into its group's span of `scratch` (with its
+// slot index beside it in the id buffer), level 2 moves it to its segment of
+// `dst`.  Level 2 runs one workgroup per chunk of an upper bound of the group
+// chunks; those past the last chunk return at once.
+template <class T>
+static ggms_status launch_segmented_first_pass(const Config& c, const Layout& L, char* W, const T* d_vec,
+                                               uint32_t n, uint32_t G0, PassCtl* cp, PlanArgs<typename KeyTraits<T>::K>& pl,
+                                               typename KeyTraits<T>::K* scratch, typename KeyTraits<T>::K* dst,
+                                               hipStream_t st, Prof& prof) {
+  using K = typename KeyTraits<T>::K;
+  using S = GroupStage<K>;
+  auto P = [&](int b) { return (void*)(W + L.off[b]); };
+  const Range<K>* rc = (const Range<K>*)P(bRangesFirst);
+  const K* lut = (const K*)P(bLut);
+  const size_t lds1 = Pass0Map<K>::param_bytes(c.R0_bound) + S::pass0_bytes(c.B0_bound);
+  LDS_CHECK(lds1 + LutGeom<K>::bytes() + 5 * 1024);
+  set_lds_attr((const void*)k_group_pass0<T>, lds1);
+  prof.begin(GGMS_KERNEL_COMPACT0, 0);
+  k_group_pass0<T><<<G0, S::kThreads, lds1, st>>>(d_vec, rc, lut, cp, (const uint32_t*)P(bRows), pl.slot_of_bucket,
+                                                  pl.slot_bucket, (uint32_t*)P(bGrpCursor), scratch,
+                                                  (uint8_t*)P(bGrpIds));
+  HIP_TRY(hipGetLastError());
+  const size_t lds2 = S::split_bytes();
+  set_lds_attr((const void*)k_group_split<K>, lds2);
+  // every group has at most one chunk beyond len / CH2 (its first starts up to
+  // N - 1 keys before the span): the chunks total at most n / CH2 + groups + 1
+  const uint32_t chunks = (uint32_t)(((uint64_t)n + S::CH2 - 1) / S::CH2) + kMaxGroups + 1;
+  k_group_split<K><<<chunks, S::kThreads2, lds2, st>>>(scratch, (const uint8_t*)P(bGrpIds), cp,
+                                                      (const uint32_t*)P(bGrpStart), pl.slot_dst, dst);
+  prof.end();
+  HIP_TRY(hipGetLastError());
+  return GGMS_OK;
+}
+
+// Plan arguments of the grouped compaction (a segmented first pass).
+template <class K>
+static void plan_groups(PlanArgs<K>& pl, const Layout& L, char* W, bool on) {
+  auto P = [&](int b) { return (void*)(W + L.off[b]); };
+  pl.grp_cursor = on ? (uint32_t*)P(bGrpCursor) : nullptr;
+  pl.grp_start = on ? (uint32_t*)P(bGrpStart) : nullptr;
 }
 
 // Status of a device-driven call from its status words.
@@ -790,7 +854,7 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
       // pass-1 range parameters live in LDS unless that would push either
       // pass-1 kernel past 112 KB (then they are read from global memory)
       uint32_t p1_lds = 0;
-      if (pass == 1) {
+      if (pass == 1 && !c.seg_first) {
         size_t cnt = ((size_t)B_bound * 4 + 15) / 16 * 16 + pass1_map_bytes<K>(B0, R0, L2, R_bound);
         size_t cmp = compact_lds_bytes(B_bound, U_slot_bound + 4) + pass1_map_bytes<K>(B0, R0, L2, R_bound);
         p1_lds = std::max(cnt, cmp) <= 112 * 1024 ? 1u : 0u;
@@ -811,7 +875,7 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
         prof.begin(GGMS_KERNEL_COUNT0, (int)pass);
         k_count0<T><<<(G + tpw - 1) / tpw, NT, lds, st>>>(d_vec, rc, (K*)P(bLut), cp, rows, tpw);
         prof.end();
-      } else if (pass == 1) {
+      } else if (pass == 1 && !c.seg_first) {
         size_t lds = ((size_t)B_bound * 4 + 15) / 16 * 16 + pass1_map_bytes<K>(B0, R0, L2, p1_lds ? R_bound : 0);
         LDS_CHECK(lds);
         set_lds_attr((const void*)k_count1<K>, lds);
@@ -840,12 +904,14 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
       pl.next_ranges = rn;
       pl.slot_of_bucket = slot_tab[pass & 1];
       pl.wg_first_next = wgf_next;
-      pl.force_finish = (c.one_pass || pass + 1 >= c.max_passes) ? 1u : c.force_finish;
-      pl.merge_small = (pass == 0 && !c.one_pass) ? 1u : 0u;
-      pl.next_params = (pass == 0 && !c.one_pass) ? (MapParam<K>*)P(bParams1) : nullptr;
-      pl.rank_words = (pass == 0 && !c.one_pass) ? (uint64_t*)P(bRank0) : nullptr;
+      pl.force_finish = (pass + 1 >= c.max_passes) ? 1u : c.force_finish;
+      pl.finish_cap = (pass == 0 && c.one_pass) ? kOnePassFinishCap * c.fin_cap : c.fin_cap;
+      pl.merge_small = (pass == 0 && !c.seg_first) ? 1u : 0u;
+      pl.next_params = (pass == 0 && !c.seg_first) ? (MapParam<K>*)P(bParams1) : nullptr;
+      pl.rank_words = (pass == 0 && !c.seg_first) ? (uint64_t*)P(bRank0) : nullptr;
       pl.mirror = dmirror;
       pl.nan_out = nullptr;
+      plan_groups<K>(pl, L, W, pass == 0 && c.seg_first);
       size_t lds_plan = (size_t)B_bound * 8 + (size_t)rank_word_count(B_bound) * 8;
       LDS_CHECK(lds_plan);
       set_lds_attr((const void*)k_plan<T>, lds_plan);
@@ -855,15 +921,10 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
       HIP_TRY(hipGetLastError());
       HIP_TRY(hipEventRecord(ctx->plan_ev, st));
       // compaction
-      if (pass == 0 && c.one_pass) {
+      if (pass == 0 && c.seg_first) {
         // every k-bucket into its own segment (the finish reads segments)
-        size_t lds = Pass0Map<K>::param_bytes(R_bound) + compact_lds_bytes(B_bound, U_slot_bound + 4);
-        LDS_CHECK(lds + LutGeom<K>::bytes());
-        set_lds_attr((const void*)k_compact0_seg<T>, lds);
-        prof.begin(GGMS_KERNEL_COMPACT0, (int)pass);
-        k_compact0_seg<T><<<G, NT, lds, st>>>(d_vec, rc, (K*)P(bLut), cp, rows, pl.slot_of_bucket, pl.slot_dst,
-                                              pl.slot_bucket, dst);
-        prof.end();
+        s = launch_segmented_first_pass<T>(c, L, W, d_vec, n, G, cp, pl, src, dst, st, prof);
+        if (s) return s;
       } else if (pass == 0) {
         // dynamic part: map parameters then the slot table (cell table is static LDS)
         size_t lds = (size_t)R_bound * sizeof(MapParam<K>) + (size_t)B_bound * 2;
@@ -873,7 +934,7 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
         k_compact0<T><<<G, NT, lds, st>>>(d_vec, rc, (K*)P(bLut), cp, rows, pl.slot_of_bucket, pl.slot_bucket,
                                           dst);
         prof.end();
-      } else if (pass == 1) {
+      } else if (pass == 1 && !c.seg_first) {
         size_t lds = compact_lds_bytes(B_bound, U_slot_bound + 4) + pass1_map_bytes<K>(B0, R0, L2, p1_lds ? R_bound : 0);
         // one workgroup per CU: its writes scatter over every k-bucket segment,
         // and twice the workgroups in flight keep twice the partial HBM lines
@@ -902,7 +963,7 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
       const PassCtl h = hmirror[0];
       const PassCtl hn = hmirror[1];
       // the iterative first pass hands every k-bucket to pass 1
-      if ((pass > 0 || c.one_pass) && h.U_small > 0) {
+      if ((pass > 0 || c.seg_first) && h.U_small > 0) {
         // every segment within a quarter of the capacity: half-size workgroups
         // (512 threads, half the LDS) so twice as many run at once
         const bool half = h.small_max <= c.fin_cap / 4;

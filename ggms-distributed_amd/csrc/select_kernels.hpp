@@ -24,6 +24,8 @@
 // copyElements_iterative :611-656, updatePivots_iterative :715-758,
 // generateBucketsandSlopes_iterative :404-477, generatePivots :1011-1054.
 #pragma once
+#include <type_traits>
+
 #include "ggms_device.hpp"
 #include "philox.hpp"
 
@@ -31,6 +33,10 @@ namespace ggms {
 
 constexpr uint32_t kNone16 = 0xFFFFu;
 constexpr uint32_t kNone32 = 0xFFFFFFFFu;
+// grouped (segmented) first-pass compaction: consecutive slots per group, and
+// a bound on the groups (above (kMaxBuckets + edge and pivot buckets) / kGroupSlots)
+constexpr uint32_t kGroupSlots = 128;
+constexpr uint32_t kMaxGroups = 256;
 
 enum KState : uint32_t { kLive = 0, kSmall = 1, kDone = 2, kSmallNew = 3 };
 enum ActClass : uint32_t { kResolved = 0, kClsSmall = 1, kClsBig = 2 };
@@ -61,7 +67,13 @@ struct PassCtl {
   uint32_t U_wide;         // small segments selected from global memory (the last U_wide of the finish order)
   uint32_t U_wave;         // small segments finished one per wave (the U_wave before those)
   uint32_t small_max;      // longest small segment of the pass
-  uint32_t pad[11];
+  // first pass: window of the key-prefix cell table, [lut_base, lut_base +
+  // (cells << lut_shift)); lut_shift == key bits - level-1 bits is the whole key space
+  uint32_t lut_base_lo, lut_base_hi, lut_shift;
+  // ranges of the keys outside that window: those below it lie in ranges
+  // [0, lut_r_below], those above it in [lut_r_above, R - 1]
+  uint32_t lut_r_below, lut_r_above;
+  uint32_t pad[6];
 };
 
 constexpr uint32_t kSetupKsPerThread = 8;
@@ -795,11 +807,27 @@ template <class K> __device__ __forceinline__ uint32_t param_bucket(const MapPar
 template <class K> struct Pass0Map {
   using G = LutGeom<K>;
   K* lut;  // level 1 followed by the level-2 tables
+  // cell table window (WIN lookups): cell c covers keys [wbase + (c << wsh),
+  // wbase + ((c + 1) << wsh)); keys outside [wbase, wbase + wmax] take the
+  // binary-search path.  Without WIN the cells are the top key bits (wsh = S1).
+  K wbase = 0;
+  K wmax = (K)~(K)0;
+  uint32_t wsh = G::S1;
+  uint32_t r_below = 0, r_above = 0;  // search spans of keys below / above the window
+  __device__ void set_window(const PassCtl* c0) {
+    wbase = sizeof(K) == 4 ? (K)c0->lut_base_lo : (K)(((uint64_t)c0->lut_base_hi << 32) | c0->lut_base_lo);
+    wsh = c0->lut_shift;
+    wmax = wsh >= G::S1 ? (K)~(K)0 : (K)((((K)G::L1CELLS) << wsh) - 1);
+    r_below = c0->lut_r_below;
+    r_above = c0->lut_r_above;
+  }
+  __device__ bool windowed() const { return wsh < G::S1; }
   K* plo;
   uint32_t* pmult;
   uint32_t* pbase;  // 64-bit keys
   uint32_t* psh;    // 64-bit keys
   uint64_t* pc;     // 32-bit keys
+  uint32_t Rn = 0;  // ranges
 
   // LDS bytes of the parameter arrays for R ranges (16 bytes per range for
   // 32-bit keys, 20 for 64-bit keys), rounded to 16
@@ -812,6 +840,7 @@ template <class K> struct Pass0Map {
   static __host__ __device__ size_t bytes(uint32_t R, uint32_t l2) { return lut_bytes(l2) + param_bytes(R); }
   // p must be 8-byte aligned
   __device__ void set_params(unsigned char* p, uint32_t R) {
+    Rn = R;
     plo = reinterpret_cast<K*>(p);
     pmult = reinterpret_cast<uint32_t*>(p + (size_t)R * sizeof(K));
     if constexpr (sizeof(K) == 4) {
@@ -868,66 +897,101 @@ template <class K> struct Pass0Map {
   // stored start offset.  Splicing the key's low bits under the entry's high
   // bits turns that test into one compare of whole words.  Meaningless for
   // non-leaf entries, which the callers overwrite.
-  static __device__ __forceinline__ uint32_t leaf_range(K e, K key) {
-    const K spliced = (key & G::M1) | (e & (K)~G::M1);
+  // Range of a key in a level-1 leaf entry: r0 (the bits above S1; the flag
+  // bit is 0 for a leaf) plus one when the key's in-cell offset passes the
+  // stored start offset.  Splicing the in-cell offset under the entry's high
+  // bits turns that test into one compare of whole words.  Meaningless for
+  // non-leaf entries, which the callers overwrite.
+  static __device__ __forceinline__ uint32_t leaf_range(K e, K off) {
+    const K spliced = off | (e & (K)~G::M1);
     return (uint32_t)(e >> G::S1) + (spliced > e ? 1u : 0u);
   }
-  __device__ __forceinline__ uint32_t range_of(K key) const {
-    K e = lut[(uint32_t)(key >> G::S1)];
-    uint32_t r = leaf_range(e, key);
-    bool nl = (e & G::FLAG) != 0;
-    if (wave_ballot(nl)) {
-      if (nl) {
-        if (e & G::TABLE) {
-          e = lut[G::L1CELLS + (((uint32_t)(e & (G::TABLE - 1))) << 6) + ((uint32_t)(key >> G::S2) & 63u)];
-          r = (uint32_t)((e >> G::S2) & 0x1FFFF) + ((key & G::M2) > (e & G::M2) ? 1u : 0u);
+  // Level-1 cell of a key, its offset inside the cell and whether it lies
+  // outside the window.
+  template <bool WIN> __device__ __forceinline__ uint32_t cell_of(K key, K& off, bool& out) const {
+    if constexpr (WIN) {
+      const K d = (K)(key - wbase);
+      out = d > wmax;
+      off = d & (K)(((K)1 << wsh) - 1);
+      return out ? 0u : (uint32_t)(d >> wsh);
+    } else {
+      out = false;
+      off = key & G::M1;
+      return (uint32_t)(key >> G::S1);
+    }
+  }
+  // The rare lookups: a level-2 table, a multi-start (sub)cell or a key
+  // outside the window (binary search over the range starts).
+  template <bool WIN> __device__ __forceinline__ uint32_t range_slow(K key, K e, bool out, uint32_t r) const {
+    uint32_t lo = 0, hi = 0;
+    bool search = out;
+    if (out) {  // the few ranges outside the window
+      const bool below = key < wbase;
+      lo = below ? 0u : r_above;
+      hi = below ? r_below : (uint32_t)(Rn - 1);
+    } else {
+      if (e & G::TABLE) {
+        uint32_t sub;
+        K soff;
+        if constexpr (WIN) {
+          const K d = (K)(key - wbase);
+          sub = (uint32_t)(d >> (wsh - 6)) & 63u;
+          soff = d & (K)(((K)1 << (wsh - 6)) - 1);
+        } else {
+          sub = (uint32_t)(key >> G::S2) & 63u;
+          soff = key & G::M2;
         }
-        if (e & G::FLAG) {
-          uint32_t lo = (uint32_t)(e & 0x7FF), hi = (uint32_t)((e >> 11) & 0x7FF);
-          while (lo < hi) {
-            uint32_t mid = (lo + hi + 1) >> 1;
-            if (plo[mid] <= key) lo = mid; else hi = mid - 1;
-          }
-          r = lo;
-        }
+        e = lut[G::L1CELLS + (((uint32_t)(e & (G::TABLE - 1))) << 6) + sub];
+        r = (uint32_t)((e >> G::S2) & 0x1FFFF) + (soff > (e & G::M2) ? 1u : 0u);
       }
+      if (e & G::FLAG) {
+        search = true;
+        lo = (uint32_t)(e & 0x7FF);
+        hi = (uint32_t)((e >> 11) & 0x7FF);
+      }
+    }
+    if (search) {
+      while (lo < hi) {
+        uint32_t mid = (lo + hi + 1) >> 1;
+        if (plo[mid] <= key) lo = mid; else hi = mid - 1;
+      }
+      r = lo;
     }
     return r;
   }
-  __device__ __forceinline__ uint32_t bucket(K key) const { return range_bucket(range_of(key), key); }
+  template <bool WIN = false> __device__ __forceinline__ uint32_t range_of(K key) const {
+    K off;
+    bool out;
+    K e = lut[cell_of<WIN>(key, off, out)];
+    uint32_t r = leaf_range(e, off);
+    bool nl = (e & G::FLAG) != 0 || out;
+    if (wave_ballot(nl)) {
+      if (nl) r = range_slow<WIN>(key, e, out, r);
+    }
+    return r;
+  }
+  template <bool WIN = false> __device__ __forceinline__ uint32_t bucket(K key) const {
+    return range_bucket(range_of<WIN>(key), key);
+  }
 
   // Buckets of N keys: the N level-1 reads are issued together, one ballot
   // covers the rare non-leaf cells of all N, then the N parameter reads.
-  template <int N> __device__ __forceinline__ void buckets(const K (&key)[N], uint32_t (&b)[N]) const {
-    K e[N];
+  template <int N, bool WIN = false> __device__ __forceinline__ void buckets(const K (&key)[N], uint32_t (&b)[N]) const {
+    K e[N], off[N];
+    bool out[N];
 #pragma unroll
-    for (int j = 0; j < N; ++j) e[j] = lut[(uint32_t)(key[j] >> G::S1)];
+    for (int j = 0; j < N; ++j) e[j] = lut[cell_of<WIN>(key[j], off[j], out[j])];
     uint32_t r[N];
     bool nl = false;
 #pragma unroll
     for (int j = 0; j < N; ++j) {
-      r[j] = leaf_range(e[j], key[j]);
-      nl |= (e[j] & G::FLAG) != 0;
+      r[j] = leaf_range(e[j], off[j]);
+      nl |= (e[j] & G::FLAG) != 0 || out[j];
     }
     if (wave_ballot(nl)) {
 #pragma unroll
-      for (int j = 0; j < N; ++j) {
-        K ej = e[j];
-        if (ej & G::FLAG) {
-          if (ej & G::TABLE) {
-            ej = lut[G::L1CELLS + (((uint32_t)(ej & (G::TABLE - 1))) << 6) + ((uint32_t)(key[j] >> G::S2) & 63u)];
-            r[j] = (uint32_t)((ej >> G::S2) & 0x1FFFF) + ((key[j] & G::M2) > (ej & G::M2) ? 1u : 0u);
-          }
-          if (ej & G::FLAG) {
-            uint32_t lo = (uint32_t)(ej & 0x7FF), hi = (uint32_t)((ej >> 11) & 0x7FF);
-            while (lo < hi) {
-              uint32_t mid = (lo + hi + 1) >> 1;
-              if (plo[mid] <= key[j]) lo = mid; else hi = mid - 1;
-            }
-            r[j] = lo;
-          }
-        }
-      }
+      for (int j = 0; j < N; ++j)
+        if ((e[j] & G::FLAG) || out[j]) r[j] = range_slow<WIN>(key[j], e[j], out[j], r[j]);
     }
 #pragma unroll
     for (int j = 0; j < N; ++j) b[j] = range_bucket(r[j], key[j]);
@@ -1002,31 +1066,48 @@ __device__ uint32_t build_lut(const K* starts, uint32_t R, K* __restrict__ g_lut
 }
 
 // Same table as build_lut, with every level-1 cell decided in O(1) from
-// counts instead of two binary searches: each range start is counted in its
-// cell (16-bit counters, two per word) or, when it sits exactly on the cell's
-// low edge, flagged in a bitmap; an exclusive scan of the per-cell totals then
-// gives, for cell c with P starts in earlier cells,
+// counts instead of two binary searches, over a window of the key space:
+// cell c covers [wbase + (c << wsh), wbase + ((c + 1) << wsh)) (wbase = 0,
+// wsh = S1: the top key bits, the whole key space).  Each range start inside
+// the window is counted in its cell (16-bit counters, two per word) or, when
+// it sits exactly on the cell's low edge, flagged in a bitmap; starts below
+// the window are counted once.  An exclusive scan of the per-cell totals then
+// gives, for cell c with P starts before it (below the window or in earlier
+// cells),
 //   r0 = P + edge(c) - 1 (last start <= the cell's low key),
 //   r1 = r0 + inside(c)  (last start <= the cell's high key).
+// A cell with two or more starts gets a level-2 table of 64 sub-cells when
+// its width allows (wsh >= 6) and tables are left, else a search entry.
 // cw: LDS scratch of lut_count_words<K>() words.  Returns the level-2 tables
 // in use (same value in every thread).
 template <class K> __host__ __device__ constexpr uint32_t lut_count_words() {
   return LutGeom<K>::L1CELLS / 2 + LutGeom<K>::L1CELLS / 32;
 }
 template <class K>
-__device__ uint32_t build_lut_counted(const K* starts, uint32_t R, K* __restrict__ g_lut, uint32_t* cw) {
+__device__ uint32_t build_lut_counted(const K* starts, uint32_t R, K* __restrict__ g_lut, uint32_t* cw,
+                                      K wbase = 0, uint32_t wsh = LutGeom<K>::S1) {
   using G = LutGeom<K>;
   constexpr uint32_t C = G::L1CELLS;
   uint32_t* edge = cw + C / 2;
   __shared__ uint32_t tab_cell[G::L2TABLES];
   __shared__ uint32_t scan_s[32];
+  __shared__ uint32_t below_s;
   const uint32_t nt = blockDim.x, t = threadIdx.x;
+  const K wmask = wsh >= G::S1 ? G::M1 : (K)(((K)1 << wsh) - 1);
+  const K wmax = wsh >= G::S1 ? (K)~(K)0 : (K)((((K)C) << wsh) - 1);
   for (uint32_t i = t; i < lut_count_words<K>(); i += nt) cw[i] = 0;
+  if (t == 0) below_s = 0;
   __syncthreads();
   for (uint32_t r = t; r < R; r += nt) {
     const K st = starts[r];
-    const uint32_t c = (uint32_t)(st >> G::S1);
-    if (st & G::M1) atomicAdd(&cw[c >> 1], 1u << (16 * (c & 1)));
+    if (st < wbase) {
+      atomicAdd(&below_s, 1u);
+      continue;
+    }
+    const K d = (K)(st - wbase);
+    if (d > wmax) continue;  // above the window: in no cell
+    const uint32_t c = (uint32_t)(d >> wsh);
+    if (d & wmask) atomicAdd(&cw[c >> 1], 1u << (16 * (c & 1)));
     else atomicOr(&edge[c >> 5], 1u << (c & 31));
   }
   __syncthreads();
@@ -1042,17 +1123,18 @@ __device__ uint32_t build_lut_counted(const K* starts, uint32_t R, K* __restrict
   }
   uint32_t tot;
   const uint32_t ex = block_excl_scan_u32((nmulti << 16) | nst, scan_s, &tot);  // both fit 16 bits
-  uint32_t P = ex & 0xFFFFu, tix = ex >> 16;
+  uint32_t P = below_s + (ex & 0xFFFFu), tix = ex >> 16;
+  const bool tables = wsh >= 6;
   for (uint32_t c = c0; c < c1; ++c) {
     const uint32_t k = inside(c), a = on_edge(c);
     const uint32_t r0 = P + a - 1, r1 = r0 + k;
-    const K lo = (K)((K)c << G::S1);
+    const K lo = (K)(wbase + ((K)c << wsh));
     K e;
     if (k == 0) {
       e = ((K)r0 << G::S1) | G::M1;
     } else if (k == 1) {
       e = ((K)r0 << G::S1) | (K)(starts[r1] - lo - 1);
-    } else if (tix < G::L2TABLES) {
+    } else if (tables && tix < G::L2TABLES) {
       tab_cell[tix] = c;
       e = G::FLAG | G::TABLE | (K)tix;
       tix++;
@@ -1064,12 +1146,13 @@ __device__ uint32_t build_lut_counted(const K* starts, uint32_t R, K* __restrict
   }
   __syncthreads();
   // level 2: 64 sub-cells of each tabled cell (few: binary search)
-  const uint32_t nm = tot >> 16;
+  const uint32_t nm = tables ? tot >> 16 : 0u;
   const uint32_t used = nm < G::L2TABLES ? nm : G::L2TABLES;
+  const uint32_t ssh = wsh - 6;
   for (uint32_t i = t; i < used * 64; i += nt) {
     uint32_t tb = i >> 6, sub = i & 63;
     uint32_t c = tab_cell[tb];
-    K lo = (K)(((K)c << G::S1) | ((K)sub << G::S2)), hi = (K)(lo + G::M2);
+    K lo = (K)(wbase + ((K)c << wsh) + ((K)sub << ssh)), hi = (K)(lo + (((K)1 << ssh) - 1));
     uint32_t r0 = last_start_le<K>(starts, 0, R - 1, lo);
     uint32_t r1 = last_start_le<K>(starts, r0, R - 1, hi);
     K e;
@@ -1143,24 +1226,30 @@ __global__ void __launch_bounds__(1024) k_count0(const T* __restrict__ x, const 
   uint32_t* hist0 = reinterpret_cast<uint32_t*>(smem + Pass0Map<K>::param_bytes(ctl->R));
   for (uint32_t i = threadIdx.x; i < B * tiles_per_wg; i += blockDim.x) hist0[i] = 0;
   mp.load(lut, ranges, ctl->R);
+  mp.set_window(ctl);
   __syncthreads();
-  for (uint32_t h = 0; h < tiles_per_wg && w0 + h < G; ++h) {
-    uint32_t* hist = hist0 + h * B;
-    uint32_t t0 = (w0 + h) * tile;
-    uint32_t t1 = tile_end(t0, tile, n);
-    constexpr int N = Vec16<T>::N;
-    stream_tile_vec_any<T, true>(
-        x, t0, t1,
-        [&](const T (&v)[N]) {
-          K k[N];
+  auto run = [&](auto win) {
+    constexpr bool W = decltype(win)::value;
+    for (uint32_t h = 0; h < tiles_per_wg && w0 + h < G; ++h) {
+      uint32_t* hist = hist0 + h * B;
+      uint32_t t0 = (w0 + h) * tile;
+      uint32_t t1 = tile_end(t0, tile, n);
+      constexpr int N = Vec16<T>::N;
+      stream_tile_vec_any<T, true>(
+          x, t0, t1,
+          [&](const T (&v)[N]) {
+            K k[N];
 #pragma unroll
-          for (int j = 0; j < N; ++j) k[j] = KeyTraits<T>::enc(v[j]);
-          uint32_t b[N];
-          mp.template buckets<N>(k, b);
-          lds_count_n<N>(hist, b);
-        },
-        [&](T v) { lds_count(hist, mp.bucket(KeyTraits<T>::enc(v))); });
-  }
+            for (int j = 0; j < N; ++j) k[j] = KeyTraits<T>::enc(v[j]);
+            uint32_t b[N];
+            mp.template buckets<N, W>(k, b);
+            lds_count_n<N>(hist, b);
+          },
+          [&](T v) { lds_count(hist, mp.template bucket<W>(KeyTraits<T>::enc(v))); });
+    }
+  };
+  if (mp.windowed()) run(std::true_type{});
+  else run(std::false_type{});
   __syncthreads();
   for (uint32_t h = 0; h < tiles_per_wg && w0 + h < G; ++h) {
     uint32_t* row = rows + (size_t)(w0 + h) * B;
@@ -1282,6 +1371,7 @@ template <class K> struct PlanArgs {
   uint32_t* wg_first_next;   // [G_max]
   void* out;                 // T[nk]
   uint32_t fin_cap;          // largest segment the finish kernel sorts
+  uint32_t finish_cap;       // largest k-bucket finished after this pass (>= fin_cap: the rest from global memory)
   uint32_t B_req;            // bucket budget of the next pass
   uint32_t B_cap;            // hard cap (LDS)
   uint32_t G_max;            // workgroups of the streaming kernels
@@ -1292,6 +1382,10 @@ template <class K> struct PlanArgs {
   uint64_t* rank_words;      // slot rank words of this pass's buckets (or null)
   PassCtl* mirror;           // host-visible copy of ctl and ctl_next, written last (or null)
   uint32_t* nan_out;         // the first pass writes its NaN count here (or null)
+  // grouped compaction (or null): slot s belongs to group s / kGroupSlots; the
+  // slots of a group are contiguous in the output, so a group owns one span
+  uint32_t* grp_cursor;      // [groups]: the next free position of each group (advanced by the compaction)
+  uint32_t* grp_start;       // [groups + 1]: first output position of each group (total at the end)
 };
 
 template <class T>
@@ -1391,7 +1485,7 @@ __global__ void __launch_bounds__(1024) k_plan(PlanArgs<typename KeyTraits<T>::K
     uint32_t cls;
     if (klo == khi) cls = kResolved;
     else if (a.merge_small) cls = kClsBig;
-    else if (c <= a.fin_cap) cls = kClsSmall;
+    else if (c <= a.finish_cap) cls = kClsSmall;
     else cls = a.force_finish ? kClsSmall : kClsBig;
     a.act_cls[q] = cls;
     a.act_cnt[q] = c;
@@ -1498,6 +1592,19 @@ __global__ void __launch_bounds__(1024) k_plan(PlanArgs<typename KeyTraits<T>::K
   if (lane_id() == 0 && small_max) atomicMax(&sh_tot[7], small_max);
   __threadfence_block();
   __syncthreads();
+
+  // group spans of the grouped compaction: group g starts where its first slot
+  // g * kGroupSlots is laid out (slots are contiguous in slot order)
+  if (a.grp_cursor) {
+    const uint32_t total = N_big + N_small;
+    const uint32_t U_slots = U_big + U_small;
+    const uint32_t ng = (U_slots + kGroupSlots - 1) / kGroupSlots;
+    for (uint32_t g = t; g <= ng; g += nt) {
+      const uint32_t pos = g < ng ? a.slot_dst[g * kGroupSlots] : total;
+      a.grp_start[g] = pos;
+      if (g < ng) a.grp_cursor[g] = pos;
+    }
+  }
 
   // slot rank words (32 buckets each): has-a-slot bits, slot of the first
   if (a.rank_words)
@@ -1727,10 +1834,13 @@ __global__ void __launch_bounds__(1024) k_compact0(const T* __restrict__ x, cons
     (void)block_excl_scan_u32(acc, part, &tot);
     if (threadIdx.x == 0) cursor = tot ? atomicAdd(&ctl->append, tot) : 0u;
   }
+  mp.set_window(ctl);
   __syncthreads();
   uint32_t t0 = blockIdx.x * tile;
   uint32_t t1 = tile_end(t0, tile, n);
   constexpr int N = Vec16<T>::N;
+  auto run_tile = [&](auto win) {
+  constexpr bool W = decltype(win)::value;
   stream_tile_vec_any<T, true>(
       x, t0, t1,
       [&](const T (&v)[N]) {
@@ -1738,7 +1848,7 @@ __global__ void __launch_bounds__(1024) k_compact0(const T* __restrict__ x, cons
 #pragma unroll
         for (int j = 0; j < N; ++j) k[j] = KeyTraits<T>::enc(v[j]);
         uint32_t b[N];
-        mp.template buckets<N>(k, b);
+        mp.template buckets<N, W>(k, b);
         bool act[N];
         uint64_t m[N];
         uint32_t tot = 0;
@@ -1763,7 +1873,7 @@ __global__ void __launch_bounds__(1024) k_compact0(const T* __restrict__ x, cons
       },
       [&](T v) {
         K key = KeyTraits<T>::enc(v);
-        bool act = slot[mp.bucket(key)] != kNone16;
+        bool act = slot[mp.template bucket<W>(key)] != kNone16;
         uint64_t m = wave_ballot(act);
         if (m) {
           uint32_t first = (uint32_t)__builtin_ctzll(m);
@@ -1773,67 +1883,355 @@ __global__ void __launch_bounds__(1024) k_compact0(const T* __restrict__ x, cons
           if (act) out[base + mask_below(m)] = key;
         }
       });
+  };
+  if (mp.windowed()) run_tile(std::true_type{});
+  else run_tile(std::false_type{});
 }
 
-// First-pass compaction, segmented layout (the one-pass bucketMultiselect):
-// every k-bucket's elements land contiguously in its segment, as the
-// reference's copyElements does before sorting the remainder
-// (DistributedSMOS/bucketMultiselect.cu:779-793).  Workgroup w claims its run
-// of each segment with one global atomicAdd of its histogram-row count; each
-// survivor then takes its position with one LDS atomic.
+// ---------------------------------------------------------------------------
+// Segmented first pass (every k-bucket in a segment of its own, as the
+// reference's copyElements lays them out before sorting the remainder,
+// DistributedSMOS/bucketMultiselect.cu:779-793), in two levels staged
+// through LDS.
+//
+// Writing every survivor straight into its segment gives a workgroup one write
+// stream per segment, and each store instruction of a wave touches up to 64
+// cache lines.  Instead slot s belongs to group s / kGroupSlots (the slots of a
+// group are contiguous in the output, so a group owns one span of it), and
+//   level 1 (k_group_pass0) streams the input and moves every survivor to its
+//     group's span, at a run each workgroup claims from its histogram row,
+//     writing beside it (same position, one byte) the slot's index inside the
+//     group;
+//   level 2 (k_group_split) takes one chunk of one group's span per
+//     workgroup and moves every key to its slot's segment.
+// Both levels stage a chunk of keys in LDS sorted by destination -- a counting
+// sort: one LDS atomic per key gives its rank among the chunk's keys bound for
+// the same place, a scan of the counts places each destination -- and write it
+// out with consecutive lanes on consecutive addresses of one run, so a wave's
+// store touches one or two cache lines per run instead of one per key.
+// ---------------------------------------------------------------------------
+template <class K> struct GroupStage {
+  // level 1: 1024 threads, PER keys a thread per chunk
+  static constexpr uint32_t PER = sizeof(K) == 4 ? 8u : 4u;
+  static constexpr uint32_t NV = PER / Vec16<K>::N;  // 16-byte loads per thread per chunk
+  static constexpr uint32_t kThreads = 1024;
+  static constexpr uint32_t CH = kThreads * PER;     // keys per chunk
+  // level 2: 512 threads, PER2 keys a thread (measured at 2^28 fp32: 0.41 ms
+  // with 163 M survivors, against 0.43 with 1024 threads and 0.52 with 256)
+  static constexpr uint32_t PER2 = sizeof(K) == 4 ? 16u : 8u;
+  static constexpr uint32_t NV2 = PER2 / Vec16<K>::N;
+  static constexpr uint32_t kThreads2 = 512;
+  static constexpr uint32_t CH2 = kThreads2 * PER2;
+  // dynamic LDS of level 1 after the map parameters: slot table, then the
+  // staged chunk (keys, slots)
+  static __host__ __device__ size_t pass0_bytes(uint32_t B) {
+    return ((size_t)B * 2 + 15) / 16 * 16 + (size_t)CH * (sizeof(K) + 2);
+  }
+  static __host__ __device__ size_t split_bytes() { return (size_t)CH2 * (sizeof(K) + 4); }
+};
+
+// r[j] = rank of key j among the keys of the workgroup with the same
+// destination d[j] so far (cnt[d] counts them; d == kNone32: no key).  A
+// destination shared by at least kPeelMin lanes of the wave (checked on the
+// first key) takes one atomic per wave.
+template <int N>
+__device__ __forceinline__ void stage_rank(uint32_t* cnt, const uint32_t (&d)[N], uint32_t (&r)[N]) {
+  const uint64_t a0 = wave_ballot(d[0] != kNone32);
+  const uint32_t d0 = a0 ? readlane_u32(d[0], (uint32_t)__builtin_ctzll(a0)) : kNone32;
+  const bool peel = a0 && __builtin_popcountll(wave_ballot(d[0] == d0)) >= kPeelMin;
+  if (peel) {
+#pragma unroll
+    for (int j = 0; j < N; ++j) r[j] = lds_claim(cnt, d[j], d[j] != kNone32);
+  } else {
+#pragma unroll
+    for (int j = 0; j < N; ++j) r[j] = d[j] != kNone32 ? atomicAdd(&cnt[d[j]], 1u) : 0u;
+  }
+}
+
+// Exclusive scan of cnt[0, m) (m <= 256) by one whole wave: off[i] gets the
+// prefix (written by this wave; waves that scan the same counts write the
+// same values), the return value is the sum.
+__device__ __forceinline__ uint32_t wave_scan_256(const uint32_t* cnt, uint32_t m, uint32_t* off) {
+  const uint32_t l = lane_id();
+  uint32_t c[4], s = 0;
+#pragma unroll
+  for (uint32_t q = 0; q < 4; ++q) {
+    c[q] = 4 * l + q < m ? cnt[4 * l + q] : 0u;
+    s += c[q];
+  }
+  const uint32_t inc = wave_incl_scan_u32(s);
+  uint32_t e = inc - s;
+#pragma unroll
+  for (uint32_t q = 0; q < 4; ++q) {
+    if (4 * l + q < m) off[4 * l + q] = e;
+    e += c[q];
+  }
+  return readlane_u32(inc, 63);
+}
+
 template <class T>
-__global__ void __launch_bounds__(1024) k_compact0_seg(const T* __restrict__ x,
-                                                       const Range<typename KeyTraits<T>::K>* __restrict__ ranges,
-                                                       const typename KeyTraits<T>::K* __restrict__ lut,
-                                                       const PassCtl* __restrict__ ctl, const uint32_t* __restrict__ rows,
-                                                       const uint16_t* __restrict__ slot_of_bucket,
-                                                       uint32_t* __restrict__ slot_dst,
-                                                       const uint32_t* __restrict__ slot_bucket,
-                                                       typename KeyTraits<T>::K* __restrict__ out) {
+__global__ void __launch_bounds__(1024) k_group_pass0(const T* __restrict__ x,
+                                                      const Range<typename KeyTraits<T>::K>* __restrict__ ranges,
+                                                      const typename KeyTraits<T>::K* __restrict__ lut,
+                                                      const PassCtl* __restrict__ ctl, const uint32_t* __restrict__ rows,
+                                                      const uint16_t* __restrict__ slot_of_bucket,
+                                                      const uint32_t* __restrict__ slot_bucket,
+                                                      uint32_t* __restrict__ grp_cursor,
+                                                      typename KeyTraits<T>::K* __restrict__ out,
+                                                      uint8_t* __restrict__ out_id) {
   using K = typename KeyTraits<T>::K;
+  using S = GroupStage<K>;
+  constexpr int N = Vec16<T>::N;
   extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ uint32_t s_cnt[2][kMaxGroups];  // keys of the chunk per group, by chunk parity
+  __shared__ uint32_t s_off[kMaxGroups];     // chunk-local start of each group's keys
+  __shared__ uint32_t s_pos[kMaxGroups];     // output position of them
+  __shared__ uint32_t s_run[kMaxGroups];     // next position of this workgroup's run of each group
+  __shared__ uint32_t s_tot;
   const uint32_t B = ctl->B, n = ctl->n, tile = ctl->tile, U_slots = ctl->U_slots;
   if (blockIdx.x >= ctl->G || U_slots == 0) return;
+  const uint32_t ng = (U_slots + kGroupSlots - 1) / kGroupSlots;
+  const uint32_t nt = blockDim.x, t = threadIdx.x;
   __shared__ __align__(16) K s_lut[LutGeom<K>::kEntries];
   Pass0Map<K> mp;
   mp.lut = s_lut;
   mp.set_params(smem, ctl->R);
-  CompactTables ct;
-  compact_init(ct, smem + Pass0Map<K>::param_bytes(ctl->R), B, U_slots, slot_of_bucket, slot_dst, slot_bucket,
-               rows + (size_t)blockIdx.x * B);
+  unsigned char* p = smem + Pass0Map<K>::param_bytes(ctl->R);
+  uint16_t* slot = reinterpret_cast<uint16_t*>(p);
+  K* stk = reinterpret_cast<K*>(p + ((size_t)B * 2 + 15) / 16 * 16);  // staged keys
+  uint16_t* sts = reinterpret_cast<uint16_t*>(stk + S::CH);              // their slots
+  for (uint32_t i = t; i < B; i += nt) slot[i] = slot_of_bucket[i];
+  for (uint32_t g = t; g < kMaxGroups; g += nt) {
+    s_cnt[0][g] = 0;
+    s_cnt[1][g] = 0;
+    s_run[g] = 0;
+  }
   mp.load(lut, ranges, ctl->R);
+  __syncthreads();
+  // this tile's survivors per group, from its histogram row: one run per group
+  const uint32_t* row = rows + (size_t)blockIdx.x * B;
+  for (uint32_t sl = t; sl < U_slots; sl += nt) {
+    const uint32_t c = row[slot_bucket[sl]];
+    if (c) atomicAdd(&s_run[sl / kGroupSlots], c);
+  }
+  __syncthreads();
+  for (uint32_t g = t; g < ng; g += nt) {
+    const uint32_t c = s_run[g];
+    s_run[g] = c ? atomicAdd(&grp_cursor[g], c) : 0u;
+  }
+  mp.set_window(ctl);
   __syncthreads();
   const uint32_t t0 = blockIdx.x * tile;
   const uint32_t t1 = tile_end(t0, tile, n);
-  constexpr int N = Vec16<T>::N;
-  stream_tile_vec_any<T, true>(
-      x, t0, t1,
-      [&](const T (&v)[N]) {
-        K k[N];
+  const bool vec = (reinterpret_cast<uintptr_t>(x) & 15u) == 0;
+  const uint4* xv = reinterpret_cast<const uint4*>(x);
+  const uint32_t ch = nt * S::PER;  // keys per chunk (the staging holds S::CH >= ch)
+  auto run_tile = [&](auto win) {
+    constexpr bool W = decltype(win)::value;
+    uint32_t par = 0;
+    // one chunk (key q of this thread valid where ok[q]): bucket and slot,
+    // rank among the chunk's keys of the same group, stage, write out
+    auto chunk = [&](const K (&k)[S::PER], const bool (&ok)[S::PER]) {
+      uint32_t d[S::PER], sl[S::PER], r[S::PER];
 #pragma unroll
-        for (int j = 0; j < N; ++j) k[j] = KeyTraits<T>::enc(v[j]);
+      for (uint32_t u = 0; u < S::NV; ++u) {
+        K kk[N];
+#pragma unroll
+        for (int j = 0; j < N; ++j) kk[j] = k[u * N + j];
         uint32_t b[N];
-        mp.template buckets<N>(k, b);
-        uint32_t sl[N];
-        bool any = false;
+        mp.template buckets<N, W>(kk, b);
 #pragma unroll
         for (int j = 0; j < N; ++j) {
-          sl[j] = ct.slot[b[j]];
-          any |= sl[j] != kNone16;
+          const uint32_t q = u * N + j;
+          sl[q] = ok[q] ? slot[b[j]] : kNone16;
+          d[q] = sl[q] != kNone16 ? sl[q] / kGroupSlots : kNone32;
         }
-        if (wave_ballot(any)) {
+      }
+      uint32_t* cnt = s_cnt[par];
+      stage_rank<S::PER>(cnt, d, r);
+      __syncthreads();  // counts complete; the previous chunk is written out
+      if (t < 64) {
+        s_tot = wave_scan_256(cnt, ng, s_off);
+        for (uint32_t g = t; g < ng; g += 64) {
+          s_pos[g] = s_run[g];
+          s_run[g] += cnt[g];
+          s_cnt[par ^ 1][g] = 0;
+        }
+      }
+      __syncthreads();
 #pragma unroll
-          for (int j = 0; j < N; ++j)
-            if (sl[j] != kNone16) out[atomicAdd(&ct.cursor[sl[j]], 1u)] = k[j];
+      for (uint32_t q = 0; q < S::PER; ++q)
+        if (d[q] != kNone32) {
+          const uint32_t pos = s_off[d[q]] + r[q];
+          stk[pos] = k[q];
+          sts[pos] = (uint16_t)sl[q];
         }
-      },
-      [&](T v) {
-        const K key = KeyTraits<T>::enc(v);
-        const uint32_t s = ct.slot[mp.bucket(key)];
-        const bool act = s != kNone16;
-        const uint32_t p = lds_claim(ct.cursor, s, act);
-        if (act) out[p] = key;
-      });
+      __syncthreads();
+      const uint32_t tot = s_tot;
+      for (uint32_t i = t; i < tot; i += nt) {
+        const uint32_t s = sts[i], g = s / kGroupSlots;
+        const uint32_t o = s_pos[g] + (i - s_off[g]);
+        out[o] = stk[i];
+        out_id[o] = (uint8_t)(s % kGroupSlots);
+      }
+      par ^= 1u;
+    };
+    // Whole chunks from 16-byte loads issued two chunks ahead: with one
+    // workgroup per CU, four 16-byte loads per lane stay in flight through a
+    // chunk's barriers.  Every load is unconditional (the two past the last
+    // whole chunk reload it) and the three load buffers rotate by unrolling,
+    // not by copies, so the compiler waits for exactly the loads the current
+    // chunk needs.
+    const uint32_t nfull = vec ? (t1 - t0) / ch : 0u;
+    if (nfull) {
+      auto issue = [&](uint32_t c, uint4 (&q)[S::NV]) {
+        const uint32_t v0 = (t0 + (c < nfull ? c : nfull - 1) * ch) / N;
+#pragma unroll
+        for (uint32_t u = 0; u < S::NV; ++u) q[u] = ld_stream(xv + v0 + u * nt + t);
+      };
+      // chunk c from qa (loaded), qb holds chunk c + 1 (in flight), qn gets c + 2
+      auto step = [&](uint32_t c, const uint4 (&qa)[S::NV], uint4 (&qn)[S::NV]) {
+        issue(c + 2, qn);
+        K k[S::PER];
+        bool ok[S::PER];
+#pragma unroll
+        for (uint32_t u = 0; u < S::NV; ++u) {
+          T v[N];
+          Vec16<T>::unpack(qa[u], v);
+#pragma unroll
+          for (int j = 0; j < N; ++j) {
+            k[u * N + j] = KeyTraits<T>::enc(v[j]);
+            ok[u * N + j] = true;
+          }
+        }
+        chunk(k, ok);
+      };
+      uint4 qx[S::NV], qy[S::NV], qz[S::NV];
+      issue(0, qx);
+      issue(1, qy);
+      for (uint32_t c = 0; c < nfull; c += 3) {
+        step(c, qx, qz);
+        if (c + 1 >= nfull) break;
+        step(c + 1, qy, qx);
+        if (c + 2 >= nfull) break;
+        step(c + 2, qz, qy);
+      }
+    }
+    // the rest (a ragged end, or a base not 16-byte aligned): element loads
+    for (uint32_t c0 = t0 + nfull * ch; c0 < t1; c0 += ch) {
+      K k[S::PER];
+      bool ok[S::PER];
+#pragma unroll
+      for (uint32_t q = 0; q < S::PER; ++q) {
+        const uint32_t e = c0 + q * nt + t;
+        ok[q] = e < t1;
+        k[q] = KeyTraits<T>::enc(x[ok[q] ? e : t0]);
+      }
+      chunk(k, ok);
+    }
+  };
+  if (mp.windowed()) run_tile(std::true_type{});
+  else run_tile(std::false_type{});
+}
+
+// Level 2: workgroup i takes chunk i of the concatenated group chunks (group g
+// cut into chunks of at most CH keys) and moves every key to its slot's segment;
+// slot_cursor[s] is the next free position of slot s's segment (the plan sets
+// it to the segment start).  Workgroups past the last chunk return at once.
+// Launched with GroupStage<K>::kThreads2 threads (a chunk is PER2 keys a thread).
+template <class K>
+__global__ void __launch_bounds__(512) k_group_split(const K* __restrict__ in, const uint8_t* __restrict__ in_id,
+                                                     const PassCtl* __restrict__ ctl,
+                                                     const uint32_t* __restrict__ grp_start,
+                                                     uint32_t* __restrict__ slot_cursor, K* __restrict__ out) {
+  using S = GroupStage<K>;
+  constexpr uint32_t N = Vec16<K>::N;
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ uint32_t s_cnt[kGroupSlots], s_off[kGroupSlots], s_pos[kGroupSlots];
+  __shared__ uint32_t s_job[3];  // group, first position, end position
+  const uint32_t U_slots = ctl->U_slots;
+  if (U_slots == 0) return;
+  const uint32_t ng = (U_slots + kGroupSlots - 1) / kGroupSlots;
+  const uint32_t nt = blockDim.x, t = threadIdx.x;
+  K* stk = reinterpret_cast<K*>(smem);                      // staged keys
+  uint32_t* sto = reinterpret_cast<uint32_t*>(stk + S::CH2);  // their output positions
+  if (t < kGroupSlots) s_cnt[t] = 0;
+  // group g's chunks start at its span's start rounded down to a 16-byte
+  // boundary plus multiples of CH2, so no chunk covers more than CH2 / N vectors
+  if (t < 64) {  // find this workgroup's group: chunk prefix over the groups
+    uint32_t c[4], s = 0;
+#pragma unroll
+    for (uint32_t q = 0; q < 4; ++q) {
+      const uint32_t g = 4 * t + q;
+      c[q] = g < ng ? (grp_start[g + 1] - (grp_start[g] & ~(N - 1)) + S::CH2 - 1) / S::CH2 : 0u;
+      s += c[q];
+    }
+    const uint32_t inc = wave_incl_scan_u32(s);
+    uint32_t e = inc - s;
+    if (t == 0) s_job[0] = kNone32;
+    wave_sync();
+#pragma unroll
+    for (uint32_t q = 0; q < 4; ++q) {
+      const uint32_t g = 4 * t + q;
+      if (blockIdx.x >= e && blockIdx.x < e + c[q]) {
+        const uint32_t gs = grp_start[g], ge = grp_start[g + 1];
+        const uint32_t cb = (gs & ~(N - 1)) + (blockIdx.x - e) * S::CH2;
+        s_job[0] = g;
+        s_job[1] = cb > gs ? cb : gs;
+        s_job[2] = ge - cb > S::CH2 ? cb + S::CH2 : ge;
+      }
+      e += c[q];
+    }
+  }
+  __syncthreads();
+  const uint32_t g = s_job[0];
+  if (g == kNone32) return;
+  const uint32_t a = s_job[1], len = s_job[2] - s_job[1];
+  // 16-byte loads of the keys (aligned down: the vectors covering [a, a + len),
+  // keys outside it dropped) and the matching N-byte loads of their ids; the
+  // buffers are allocated in 256-byte multiples, so every vector that holds a
+  // key of the chunk lies inside them
+  using IdWord = typename std::conditional<N == 4, uint32_t, uint16_t>::type;
+  const uint32_t v0 = a / N, v1 = (a + len + N - 1) / N;
+  K k[S::PER2];
+  uint32_t d[S::PER2], r[S::PER2];
+#pragma unroll
+  for (uint32_t u = 0; u < S::NV2; ++u) {
+    const uint32_t v = v0 + u * nt + t;
+    uint4 q = make_uint4(0, 0, 0, 0);
+    IdWord w = 0;
+    if (v < v1) {
+      q = reinterpret_cast<const uint4*>(in)[v];
+      w = reinterpret_cast<const IdWord*>(in_id)[v];
+    }
+    K e[N];
+    __builtin_memcpy(e, &q, 16);
+#pragma unroll
+    for (uint32_t j = 0; j < N; ++j) {
+      const uint32_t pos = v * N + j;
+      const bool ok = v < v1 && pos >= a && pos - a < len;
+      k[u * N + j] = e[j];
+      d[u * N + j] = ok ? (uint32_t)((w >> (8 * j)) & 0xFFu) : kNone32;
+    }
+  }
+  stage_rank<S::PER2>(s_cnt, d, r);
+  __syncthreads();
+  // one claim per slot of the chunk, then every wave scans the counts itself
+  if (t < kGroupSlots) {
+    const uint32_t c = s_cnt[t];
+    s_pos[t] = c ? atomicAdd(&slot_cursor[g * kGroupSlots + t], c) : 0u;
+  }
+  (void)wave_scan_256(s_cnt, kGroupSlots, s_off);
+  __syncthreads();
+#pragma unroll
+  for (uint32_t q = 0; q < S::PER2; ++q)
+    if (d[q] != kNone32) {
+      const uint32_t pos = s_off[d[q]] + r[q];
+      stk[pos] = k[q];
+      sto[pos] = s_pos[d[q]] + r[q];
+    }
+  __syncthreads();
+  for (uint32_t i = t; i < len; i += nt) out[sto[i]] = stk[i];
 }
 
 // Pass 1 reads the append-layout buffer: an element's range is the slot of its
@@ -1859,10 +2257,12 @@ template <class K> struct Pass1Map {
   __device__ __forceinline__ uint32_t range_bucket(uint32_t s0, K key) const {
     return p1_lds ? p1s.range_bucket(s0, key) : param_bucket<K>(p1[s0], key);
   }
-  __device__ __forceinline__ uint32_t bucket(K key) const { return range_bucket(slot0(m0.bucket(key)), key); }
-  template <int N> __device__ __forceinline__ void buckets(const K (&key)[N], uint32_t (&b)[N]) const {
+  template <bool WIN = false> __device__ __forceinline__ uint32_t bucket(K key) const {
+    return range_bucket(slot0(m0.template bucket<WIN>(key)), key);
+  }
+  template <int N, bool WIN = false> __device__ __forceinline__ void buckets(const K (&key)[N], uint32_t (&b)[N]) const {
     uint32_t b0[N];
-    m0.template buckets<N>(key, b0);
+    m0.template buckets<N, WIN>(key, b0);
     uint32_t s0[N];
 #pragma unroll
     for (int j = 0; j < N; ++j) s0[j] = slot0(b0[j]);
@@ -1889,6 +2289,7 @@ __device__ __forceinline__ void pass1_map_init(Pass1Map<K>& pm, unsigned char* p
   const uint32_t B0 = ctl0->B, R0 = ctl0->R, l2 = ctl0->l2_used;
   pm.m0.carve(p, R0, l2);  // first-pass cell table first (p is LDS offset 0 in the pass-1 kernels)
   pm.m0.load(lut, rng0, R0, l2);
+  pm.m0.set_window(ctl0);
   uint64_t* rk = reinterpret_cast<uint64_t*>(p + Pass0Map<K>::bytes(R0, l2));
   for (uint32_t i = threadIdx.x; i < rank_word_count(B0); i += blockDim.x) rk[i] = rank0_g[i];
   pm.rank0 = rk;
@@ -1929,14 +2330,19 @@ __global__ void __launch_bounds__(1024) k_count1(const K* __restrict__ buf, cons
   uint32_t t0 = blockIdx.x * tile;
   uint32_t t1 = tile_end(t0, tile, n);
   constexpr int N = Vec16<K>::N;
-  stream_tile_vec<K>(
-      buf, t0, t1,
-      [&](const K (&k)[N]) {
-        uint32_t b[N];
-        pm.template buckets<N>(k, b);
-        lds_count_n<N>(hist, b);
-      },
-      [&](K key) { lds_count(hist, pm.bucket(key)); });
+  auto run_tile = [&](auto win) {
+    constexpr bool W = decltype(win)::value;
+    stream_tile_vec<K>(
+        buf, t0, t1,
+        [&](const K (&k)[N]) {
+          uint32_t b[N];
+          pm.template buckets<N, W>(k, b);
+          lds_count_n<N>(hist, b);
+        },
+        [&](K key) { lds_count(hist, pm.template bucket<W>(key)); });
+  };
+  if (pm.m0.windowed()) run_tile(std::true_type{});
+  else run_tile(std::false_type{});
   __syncthreads();
   uint32_t* row = rows + (size_t)blockIdx.x * B;
   for (uint32_t i = threadIdx.x; i < B; i += blockDim.x) row[i] = hist[i];
@@ -1965,11 +2371,13 @@ __global__ void __launch_bounds__(1024) k_compact1(const K* __restrict__ buf, co
   uint32_t t0 = blockIdx.x * tile;
   uint32_t t1 = tile_end(t0, tile, n);
   constexpr int N = Vec16<K>::N;
+  auto run_tile = [&](auto win) {
+  constexpr bool W = decltype(win)::value;
   stream_tile_vec<K>(
       buf, t0, t1,
       [&](const K (&k)[N]) {
         uint32_t b[N];
-        pm.template buckets<N>(k, b);
+        pm.template buckets<N, W>(k, b);
         uint32_t sl[N];
         bool any = false;
 #pragma unroll
@@ -1986,11 +2394,14 @@ __global__ void __launch_bounds__(1024) k_compact1(const K* __restrict__ buf, co
         }
       },
       [&](K key) {
-        uint32_t s = ct.slot[pm.bucket(key)];
+        uint32_t s = ct.slot[pm.template bucket<W>(key)];
         bool act = s != kNone16;
         uint32_t p = lds_claim(ct.cursor, s, act);
         if (act) out[p] = key;
       });
+  };
+  if (pm.m0.windowed()) run_tile(std::true_type{});
+  else run_tile(std::false_type{});
 }
 
 template <class K>

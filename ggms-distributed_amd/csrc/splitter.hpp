@@ -386,9 +386,39 @@ __global__ void __launch_bounds__(1024) k_plan0(const T* __restrict__ x, Plan0Ar
     ctl->nan_lo_bucket = KT::kFloat ? 0u : kNone32;
     ctl->nan_hi_bucket = KT::kFloat ? Btot - 1 : kNone32;
   }
-  // 5. key-prefix cell table, from the LDS copy of the range starts
-  const uint32_t l2 = build_lut_counted<K>(rlo, R, lut, cells);
-  if (threadIdx.x == 0) ctl->l2_used = l2;
+  // 5. key-prefix cell table, from the LDS copy of the range starts, over the
+  //    window the interior ranges span (sample minimum through the start of
+  //    the top edge range): narrow inputs (small integers, one sign of floats)
+  //    get cells as fine as their span allows instead of sharing a handful of
+  //    top-key-bit cells; a span of the whole key space keeps the top bits
+  using G = LutGeom<K>;
+  const K wlo = q[0], whi = q[Q - 1];
+  const uint32_t need = bitlen64((uint64_t)(K)(whi - wlo) + 1u);
+  uint32_t wsh = need > G::L1BITS ? need - G::L1BITS : 0u;
+  K wbase = wlo;
+  if (wsh >= G::S1) {
+    wsh = G::S1;
+    wbase = 0;
+  } else {
+    const K width = (K)(((K)G::L1CELLS) << wsh);
+    if ((K)(KeyMax<K>::v - wbase) < (K)(width - 1)) wbase = (K)(KeyMax<K>::v - (width - 1));
+  }
+  const uint32_t l2 = build_lut_counted<K>(rlo, R, lut, cells, wbase, wsh);
+  if (threadIdx.x == 0) {
+    ctl->l2_used = l2;
+    ctl->lut_base_lo = (uint32_t)(uint64_t)wbase;
+    ctl->lut_base_hi = (uint32_t)((uint64_t)wbase >> 32);
+    ctl->lut_shift = wsh;
+    // the ranges holding keys outside the window: below it, the ranges up to
+    // the one holding wbase - 1; above it, from the one holding the first key
+    // past the window (rlo[0] == 0: every key has a range)
+    const K wend = wsh >= G::S1 ? KeyMax<K>::v : (K)(wbase + ((((K)G::L1CELLS) << wsh) - 1));
+    uint32_t rb = 0, ra = R - 1;
+    if (wbase > 0) rb = last_start_le<K>(rlo, 0, R - 1, (K)(wbase - 1));
+    if (wend < KeyMax<K>::v) ra = last_start_le<K>(rlo, 0, R - 1, (K)(wend + 1));
+    ctl->lut_r_below = rb;
+    ctl->lut_r_above = ra;
+  }
 }
 
 }  // namespace ggms

@@ -31,7 +31,8 @@ class Opts(C.Structure):
         ("profile", C.c_uint32),
         ("schedule", C.c_uint32),
         ("device_passes", C.c_uint32),
-        ("reserved", C.c_uint32 * 5),
+        ("first_pass", C.c_uint32),
+        ("reserved", C.c_uint32 * 4),
     ]
 
 

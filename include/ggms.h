@@ -71,8 +71,20 @@ typedef struct {
   uint32_t schedule;      /* ggms_schedule (default HOST) */
   uint32_t device_passes; /* DEVICE schedule: bucketing passes launched (0 = 3, max 8); the last one finishes
                              every k-bucket left whatever its size */
-  uint32_t reserved[5];
+  uint32_t first_pass;    /* ggms_first_pass: layout of the first pass's output (iterative forms; default AUTO) */
+  uint32_t reserved[4];
 } ggms_opts;
+
+/* Layout of the first pass's output in the iterative forms (ITERATIVE and
+ * DISTRIBUTED_1GPU; the one-pass forms are always SEGMENTED).
+ *   MERGED:    every k-bucket goes on to one second pass that re-buckets them
+ *              all together (the reference's iterativeSMOS loop); few ks.
+ *   SEGMENTED: every k-bucket is compacted into a segment of its own; those
+ *              within the finish capacity are finished right away, the others
+ *              re-bucketed segment by segment; many ks.
+ *   AUTO:      SEGMENTED when there is at least one k per five first-pass
+ *              buckets, MERGED otherwise.  Results are identical either way. */
+typedef enum { GGMS_FIRST_PASS_AUTO = 0, GGMS_FIRST_PASS_MERGED = 1, GGMS_FIRST_PASS_SEGMENTED = 2 } ggms_first_pass;
 
 /* How a call's passes are driven.
  *   HOST:   the host reads one control block per pass (from pinned memory, while

@@ -131,6 +131,14 @@ int main(int argc, char** argv) {
     ps.push_back({std::vector<float>(good.begin(), good.begin() + 20000), {17, 20000}});
     uint32_t threads = argc > 1 ? atoi(argv[1]) : 1024;
     bad += run_batch(ps, 256, threads, "errors");
+    // the GPU test's batch: NaN problems beside valid ones only
+    std::vector<Problem> pn;
+    pn.push_back({good, {1, 2, 50000}});
+    pn.push_back({nan_big, {5}});
+    pn.push_back({std::vector<float>(good.begin(), good.begin() + 1000), {3}});
+    pn.push_back({nan_small, {3}});
+    pn.push_back({std::vector<float>(good.begin(), good.begin() + 20000), {17, 20000}});
+    bad += run_batch(pn, 256, 1024, "nan");
   }
   // random batches: mixed sizes, many ks, duplicates
   for (int trial = 0; trial < 3; ++trial) {

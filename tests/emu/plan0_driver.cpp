@@ -1,6 +1,7 @@
 #include "hip/hip_runtime.h"
 #include <algorithm>
 #include <random>
+#include <cmath>
 #include "splitter.hpp"
 // shfl_xor among 32-lane groups: emulate via a per-wave exchange (all lanes call)
 struct XSync { std::mutex m; std::condition_variable cv; uint32_t arrived = 0, gen = 0; float v[64]; };
@@ -22,12 +23,23 @@ int run(uint32_t mode, uint32_t S, uint32_t P, uint32_t B, int kind, uint64_t se
   std::normal_distribution<float> nd;
   uint32_t n = 1 << 20;
   std::vector<float> x(n);
-  for (auto& v : x) { float f = nd(rng); if (kind == 1) f = (rng() % 10 < 9) ? 2.5f : f; if (kind == 2) f = (float)(rng() % 5); x[&v - x.data()] = f; }
+  for (auto& v : x) {
+    float f = nd(rng);
+    if (kind == 1) f = (rng() % 10 < 9) ? 2.5f : f;
+    if (kind == 2) f = (float)(rng() % 5);
+    if (kind == 3) f = (float)(rng() % 1024);  // narrow span: fine cells
+    if (kind == 4) f = std::fabs(f);            // one sign: a window of part of the key space
+    if (kind == 5) {                             // narrow peaks: many starts per cell (level-2 tables)
+      const float mu[3] = {37.5f, -12.25f, 80.125f};
+      f = mu[rng() % 3] + 0.01f * f;
+    }
+    x[&v - x.data()] = f;
+  }
   std::vector<float> w(kMlpParams);
   for (auto& v : w) v = (float)((int)(rng() % 2001) - 1000) / 1000.0f;
   Plan0Args a{}; a.mode = mode; a.S = S; a.P = P; a.B = B; a.E = 4; a.strideS = 8192; a.seed = seed; a.mlp = w.data();
   std::vector<Range<uint32_t>> ranges(4096);
-  std::vector<uint32_t> lut(LutGeom<uint32_t>::L1CELLS + LutGeom<uint32_t>::L2TABLES * 64);
+  std::vector<uint32_t> lut(LutGeom<uint32_t>::kEntries);
   PassCtl ctl{}; ctl.n = n;
   uint32_t Sp = 1; while (Sp < S) Sp <<= 1;
   std::vector<unsigned char> smem(plan0_lds_bytes<uint32_t>(Sp) + 64);
@@ -49,7 +61,9 @@ int run(uint32_t mode, uint32_t S, uint32_t P, uint32_t B, int kind, uint64_t se
   // reference map by binary search vs LUT-based map
   std::vector<uint32_t> starts(R); for (uint32_t r = 0; r < R; ++r) starts[r] = ranges[r].lo;
   std::vector<unsigned char> prm(Pass0Map<uint32_t>::param_bytes(R) + 16);
-  Pass0Map<uint32_t> mp; mp.lut = lut.data(); mp.set_params(prm.data(), R);
+  Pass0Map<uint32_t> mp; mp.lut = lut.data(); mp.set_params(prm.data(), R); mp.set_window(&ctl);
+  const bool narrow = kind >= 2 && kind <= 4;  // kinds 2-4 hold one sign only
+  if (narrow != mp.windowed()) { printf("kind %d: window shift %u unexpected\n", kind, ctl.lut_shift); bad++; }
   for (uint32_t r = 0; r < R; ++r) mp.store_param(r, ranges[r]);
   for (int t = 0; t < 200000; ++t) {
     uint32_t key = (uint32_t)rng();
@@ -57,7 +71,12 @@ int run(uint32_t mode, uint32_t S, uint32_t P, uint32_t B, int kind, uint64_t se
     uint32_t r = std::upper_bound(starts.begin(), starts.end(), key) - starts.begin() - 1;
     uint32_t want = ranges[r].base + map_off<uint32_t>(key, ranges[r].lo, ranges[r].sh, ranges[r].mult);
     threadIdx = dim3(0);
-    uint32_t got = mp.bucket(key);
+    if (t % 4 == 3) key = starts[rng() % R] + (uint32_t)(rng() % 3) - 1;  // at and around range starts
+    if (t % 4 == 3) {
+      r = std::upper_bound(starts.begin(), starts.end(), key) - starts.begin() - 1;
+      want = ranges[r].base + map_off<uint32_t>(key, ranges[r].lo, ranges[r].sh, ranges[r].mult);
+    }
+    uint32_t got = mp.windowed() ? mp.bucket<true>(key) : mp.bucket<false>(key);
     if (got != want) { if (bad < 5) printf("key %08x got %u want %u (r=%u)\n", key, got, want, r); bad++; }
   }
   printf("mode %u S %u P %u kind %d: R=%u B=%u %s\n", mode, S, P, kind, R, ctl.B, bad ? "FAIL" : "ok");
@@ -66,7 +85,7 @@ int run(uint32_t mode, uint32_t S, uint32_t P, uint32_t B, int kind, uint64_t se
 int main() {
   int bad = 0;
 
-  for (int kind = 0; kind < 3; ++kind) {
+  for (int kind = 0; kind < 6; ++kind) {
     bad += run(kModeLinear, 1024, 17, 8192, kind, 1 + kind);
     bad += run(kModeSketch, 8192, 256, 16384, kind, 11 + kind);
     bad += run(kModeLearned, 8192, 256, 16384, kind, 21 + kind);

@@ -355,3 +355,26 @@ def test_oversized_segments_finish_from_global_memory(gpu_ctx, dt, dist, kdist, 
         bad = np.nonzero(_bits(got) != _bits(want))[0]
         assert bad.size == 0, f"{alg}: {bad.size} wrong, first k={ks[bad[0]]}"
         print(f"{dt}/{dist}/{kdist} {alg}: exact", flush=True)
+
+
+@pytest.mark.parametrize("dt,dist,p0", [("float32", "mixture", 0), ("float64", "normal", 0), ("uint32", "uniform", 1024),
+                                        ("float32", "near_constant", 1.0), ("int32", "uniform", 0)])
+def test_first_pass_layouts(gpu_ctx, dt, dist, p0):
+    """The iterative forms with each first-pass layout: MERGED (every k-bucket
+    re-bucketed together in pass 2) and SEGMENTED (every k-bucket compacted into
+    a segment of its own by the two-level grouped compaction; those within the
+    finish capacity finished at once, the larger ones re-bucketed segment by
+    segment), with few and many ks, small bucket budgets (long re-bucketing
+    chains below the first pass) and a two-pass cap (forced finish)."""
+    import ggms
+    n = (1 << 23) + 5
+    x = ggms.generate(dt, dist, n, 41, p0, 0.01 if dist == "near_constant" else 0)
+    for kdist, nk in (("uniform_random", 1001), ("even", 8192), ("window", 5000)):
+        ks = ggms.generate_ks(kdist, nk, n, 42 + nk)
+        want = oracle.sort_and_choose(_host(x), ks, threads=16)
+        for alg in ("iterative", "distributed"):
+            for fp, extra in ((1, {}), (2, {}), (2, {"num_buckets": 64}), (2, {"num_buckets": 512, "max_passes": 2})):
+                got = gpu_ctx.select(x, ks, alg, ggms.default_opts(first_pass=fp, **extra))
+                bad = np.nonzero(_bits(got) != _bits(want))[0]
+                assert bad.size == 0, f"{kdist}/{nk} {alg} first_pass={fp} {extra}: {bad.size} wrong, first k={ks[bad[0]]}"
+        print(f"{dt}/{dist}/{kdist}: exact", flush=True)

@@ -105,7 +105,7 @@ def test_finish_multi_rank_radix_select(gen_dir):
 
 def test_first_pass_splitter_ranges_and_cell_table(gen_dir):
     out = _build_and_run(gen_dir, "plan0_driver.cpp")
-    assert out.count(" ok") == 12
+    assert out.count(" ok") == 24
 
 
 def test_batched_kernel_paths(gen_dir):
@@ -113,4 +113,4 @@ def test_batched_kernel_paths(gen_dir):
     mixed batches with duplicates, and tiny first-pass tables that force the
     re-bucketing work stack."""
     out = _build_and_run(gen_dir, "batch_driver.cpp", [256])
-    assert out.count(", ok") == 4
+    assert out.count(", ok") == 5
