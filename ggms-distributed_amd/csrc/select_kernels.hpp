@@ -1007,6 +1007,17 @@ template <class K> __device__ __forceinline__ uint32_t last_start_le(const K* st
   return lo;
 }
 
+// Level-2 entry of a sub-cell with low key lo whose last range starting at
+// or below lo is r0 and last starting at or below its high key r1: a leaf
+// (at most one start inside) or a search span.
+template <class K>
+__device__ __forceinline__ K l2_entry(const K* starts, uint32_t r0, uint32_t r1, K lo) {
+  using G = LutGeom<K>;
+  if (r1 == r0) return ((K)r0 << G::S2) | G::M2;
+  if (r1 == r0 + 1) return ((K)r0 << G::S2) | (K)(starts[r1] - lo - 1);
+  return G::FLAG | (K)r0 | ((K)r1 << 11);
+}
+
 // Two-level table of a range list given its sorted starts (in LDS; the ranges
 // partition the key space, starts[0] == 0, R <= 2047).  Called by a whole
 // workgroup; writes level 1 and the level-2 tables to g_lut and returns the
@@ -1056,11 +1067,7 @@ __device__ uint32_t build_lut(const K* starts, uint32_t R, K* __restrict__ g_lut
     K lo = (K)(((K)c << G::S1) | ((K)sub << G::S2)), hi = (K)(lo + G::M2);
     uint32_t r0 = last_start_le<K>(starts, 0, R - 1, lo);
     uint32_t r1 = last_start_le<K>(starts, r0, R - 1, hi);
-    K e;
-    if (r1 == r0) e = ((K)r0 << G::S2) | G::M2;
-    else if (r1 == r0 + 1) e = ((K)r0 << G::S2) | (K)(starts[r1] - lo - 1);
-    else e = G::FLAG | (K)r0 | ((K)r1 << 11);
-    g_lut[G::L1CELLS + i] = e;
+    g_lut[G::L1CELLS + i] = l2_entry<K>(starts, r0, r1, lo);
   }
   return used;
 }
@@ -1155,11 +1162,7 @@ __device__ uint32_t build_lut_counted(const K* starts, uint32_t R, K* __restrict
     K lo = (K)(wbase + ((K)c << wsh) + ((K)sub << ssh)), hi = (K)(lo + (((K)1 << ssh) - 1));
     uint32_t r0 = last_start_le<K>(starts, 0, R - 1, lo);
     uint32_t r1 = last_start_le<K>(starts, r0, R - 1, hi);
-    K e;
-    if (r1 == r0) e = ((K)r0 << G::S2) | G::M2;
-    else if (r1 == r0 + 1) e = ((K)r0 << G::S2) | (K)(starts[r1] - lo - 1);
-    else e = G::FLAG | (K)r0 | ((K)r1 << 11);
-    g_lut[G::L1CELLS + i] = e;
+    g_lut[G::L1CELLS + i] = l2_entry<K>(starts, r0, r1, lo);
   }
   return used;
 }

@@ -307,6 +307,11 @@ __global__ void __launch_bounds__(1024) k_plan0(const T* __restrict__ x, Plan0Ar
   // interior range.  Runs are found in parallel (a run starts where the pivot
   // changes) and placed by a block scan; NaN and edge ranges bracket them.
   __shared__ uint32_t scan_tmp[32];
+  // longest run of equal pivots (length - 1) << 16 | its first pivot: the
+  // most frequent sample value, which the cell-table window puts on a cell edge
+  __shared__ uint32_t hot_run;
+  if (threadIdx.x == 0) hot_run = 0;
+  __syncthreads();
   {
     enum : uint32_t { kSingle = 0, kEdge = 1, kInterior = 2, kNan = 3 };
     const uint32_t pre = (KT::kFloat ? 1u : 0u) + (q[0] > lowest ? 1u : 0u);
@@ -317,6 +322,7 @@ __global__ void __launch_bounds__(1024) k_plan0(const T* __restrict__ x, Plan0Ar
       if (i > 0 && q[i] == q[i - 1]) continue;
       uint32_t j = i;
       while (j + 1 < Q && q[j + 1] == q[i]) ++j;
+      if (j > i) atomicMax(&hot_run, ((j - i) << 16) | i);
       if (j + 1 >= Q) cnt += 1;
       else if (j > i) {
         cnt += 1;
@@ -396,6 +402,25 @@ __global__ void __launch_bounds__(1024) k_plan0(const T* __restrict__ x, Plan0Ar
   const uint32_t need = bitlen64((uint64_t)(K)(whi - wlo) + 1u);
   uint32_t wsh = need > G::L1BITS ? need - G::L1BITS : 0u;
   K wbase = wlo;
+  if (wsh < G::S1 && hot_run) {
+    // a repeated pivot value v is a one-key range [v, v] followed by a range
+    // starting at v + 1: with v on a cell's low edge that cell is a one-start
+    // leaf instead of a level-2 table, so the (many) keys equal to v take the
+    // one-compare path.  Move the window's base down to the cell edge below v
+    // (one cell more when the span then overflows the window).
+    const K v = q[hot_run & 0xFFFFu];
+    for (uint32_t w = wsh; w <= wsh + 1 && w < G::S1; ++w) {
+      const K cw = (K)((K)1 << w);
+      const K below = (K)(((K)(v - wlo) + cw - 1) / cw * cw);  // v - base, a multiple of the cell width
+      if (below > v) break;                                      // no room below v
+      const K base = (K)(v - below);
+      if ((K)(whi - base) <= (K)((((K)G::L1CELLS) << w) - 1)) {
+        wbase = base;
+        wsh = w;
+        break;
+      }
+    }
+  }
   if (wsh >= G::S1) {
     wsh = G::S1;
     wbase = 0;

@@ -6,8 +6,8 @@
 //   * __syncthreads is a barrier of the caller's workgroup that also checks
 //     every work-item arrives at the same call site, and that none has left
 //     the kernel while others wait;
-//   * "one-lane waves": a lane's ballot is its own predicate, so the kernels'
-//     wave-aggregated LDS atomics take their per-lane path;
+//   * "one-lane waves": a lane's ballot is its own predicate (its own bit),
+//     so the kernels' wave-aggregated LDS atomics take their per-lane path;
 //   * wave-wide scans and shuffles (called by full waves only) are emulated
 //     across the 64 threads of a wave;
 //   * __shared__ variables become per-workgroup slots (emu_shared) that a
@@ -129,7 +129,7 @@ template <class T> inline T atomicMax(T* p, T v) {
   return cur;
 }
 inline uint32_t __umulhi(uint32_t a, uint32_t b) { return (uint32_t)(((uint64_t)a * b) >> 32); }
-inline uint64_t __ballot(int p) { return p ? 1ull : 0ull; }
+inline uint64_t __ballot(int p) { return p ? 1ull << (threadIdx.x & 63) : 0ull; }  // the lane's own bit
 inline uint32_t __lane_id() { return threadIdx.x & 63; }
 inline int __builtin_amdgcn_readlane(int v, int) { return v; }
 inline int __builtin_amdgcn_readfirstlane(int v) { return v; }

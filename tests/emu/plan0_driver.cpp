@@ -29,6 +29,7 @@ int run(uint32_t mode, uint32_t S, uint32_t P, uint32_t B, int kind, uint64_t se
     if (kind == 2) f = (float)(rng() % 5);
     if (kind == 3) f = (float)(rng() % 1024);  // narrow span: fine cells
     if (kind == 4) f = std::fabs(f);            // one sign: a window of part of the key space
+    if (kind == 6) f = (rng() % 100 < 99) ? 1.0f : f;  // 99 % one value, 1 % N(0, 1)
     if (kind == 5) {                             // narrow peaks: many starts per cell (level-2 tables)
       const float mu[3] = {37.5f, -12.25f, 80.125f};
       f = mu[rng() % 3] + 0.01f * f;
@@ -63,7 +64,15 @@ int run(uint32_t mode, uint32_t S, uint32_t P, uint32_t B, int kind, uint64_t se
   std::vector<unsigned char> prm(Pass0Map<uint32_t>::param_bytes(R) + 16);
   Pass0Map<uint32_t> mp; mp.lut = lut.data(); mp.set_params(prm.data(), R); mp.set_window(&ctl);
   const bool narrow = kind >= 2 && kind <= 4;  // kinds 2-4 hold one sign only
-  if (narrow != mp.windowed()) { printf("kind %d: window shift %u unexpected\n", kind, ctl.lut_shift); bad++; }
+  if (kind <= 5 && narrow != mp.windowed()) { printf("kind %d: window shift %u unexpected\n", kind, ctl.lut_shift); bad++; }
+  if (kind == 6 && mp.windowed()) {  // the dominant value sits on a cell edge: a leaf, not a table
+    const uint32_t d = KeyTraits<float>::enc(1.0f) - ctl.lut_base_lo;
+    const uint32_t e = lut[d >> ctl.lut_shift];
+    if ((d & ((1u << ctl.lut_shift) - 1)) != 0 || (e & LutGeom<uint32_t>::FLAG)) {
+      printf("kind 6: dominant value at cell offset %u, entry %08x\n", d & ((1u << ctl.lut_shift) - 1), e);
+      bad++;
+    }
+  }
   for (uint32_t r = 0; r < R; ++r) mp.store_param(r, ranges[r]);
   for (int t = 0; t < 200000; ++t) {
     uint32_t key = (uint32_t)rng();
@@ -85,7 +94,7 @@ int run(uint32_t mode, uint32_t S, uint32_t P, uint32_t B, int kind, uint64_t se
 int main() {
   int bad = 0;
 
-  for (int kind = 0; kind < 6; ++kind) {
+  for (int kind = 0; kind < 7; ++kind) {
     bad += run(kModeLinear, 1024, 17, 8192, kind, 1 + kind);
     bad += run(kModeSketch, 8192, 256, 16384, kind, 11 + kind);
     bad += run(kModeLearned, 8192, 256, 16384, kind, 21 + kind);

@@ -89,8 +89,8 @@ def test_learned_splitter_is_exact_with_any_weights(gpu_ctx, dt, dist):
                     reason="trained weights not committed yet")
 def test_trained_network_gain(gpu_ctx):
     """Mixtures of 1-8 Gaussians + lognormal tails, 2^13 ks: the learned first pass keeps fewer
-    elements than the reference linear splitter, with identical exact outputs; the sketch
-    splitter (same sample and pieces, no network) is reported beside it."""
+    elements than the reference linear splitter and than the sketch splitter (same sample and
+    pieces, no network: what the trained network itself gains), with identical exact outputs."""
     import ggms
     from ggms import learned
     learned.install(0)
@@ -108,3 +108,6 @@ def test_trained_network_gain(gpu_ctx):
     tot = {k: sum(v) for k, v in kept.items()}
     print("first-pass survivors:", tot)
     assert tot["learned"] < tot["linear"]
+    # the network's own contribution: same sample and piece count as the sketch,
+    # pivots predicted instead of read off the sample (a small margin: ~0.2 %)
+    assert tot["learned"] < tot["sketch"]

@@ -105,7 +105,7 @@ def test_finish_multi_rank_radix_select(gen_dir):
 
 def test_first_pass_splitter_ranges_and_cell_table(gen_dir):
     out = _build_and_run(gen_dir, "plan0_driver.cpp")
-    assert out.count(" ok") == 24
+    assert out.count(" ok") == 28
 
 
 def test_batched_kernel_paths(gen_dir):
