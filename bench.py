@@ -66,6 +66,7 @@ def bench_other_configs(ctx, args):
         x = ggms.generate(dt, dist, n, args.seed + lg, p0, p1)
         ks = ggms.generate_ks(kd, nk, n, args.seed + nk)
         ref = None
+        ms_linear = None
         for sp in splitters:
             o = ggms.default_opts(splitter=sp)
             got = ctx.select(x, ks, "iterative", o)
@@ -77,9 +78,11 @@ def bench_other_configs(ctx, args):
                 ctx.select(x, ks, "iterative", o)
             torch.cuda.synchronize()
             ms = (time.perf_counter() - t0) / reps * 1e3
+            ms_linear = ms if sp == "linear" else ms_linear
             ctx.select(x, ks, "iterative", ggms.default_opts(splitter=sp, profile=1))
             st = ctx.stats()
             out.append({"workload": f"{label}, N=2^{lg}, {nk} ks {kd}", "splitter": sp, "ms_per_step": round(ms, 4),
+                        "speedup_vs_linear": round(ms_linear / ms, 4) if ms_linear else None,
                         "elements_per_s": round(n / (ms * 1e-3), 1), "passes": st["passes"],
                         "n_kept": st["n_kept"], "buckets": st["buckets"],
                         "identical_to_linear": bool((got.view(np.uint8) == ref.view(np.uint8)).all()),

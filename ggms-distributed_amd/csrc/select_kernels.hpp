@@ -1547,12 +1547,26 @@ __global__ void __launch_bounds__(1024) k_plan(PlanArgs<typename KeyTraits<T>::K
   uint32_t* sinfo = P + U_small;  // class information per small segment
   uint32_t* kst = aidx;           // k-list start per small segment (from P8 on)
   uint32_t small_max = 0;  // this thread's longest small segment
-  for (uint32_t q = t; q < U; q += nt) {
-    uint32_t s = a.act_slot[q];
+  // four active buckets per thread at a time, their loads issued together
+  for (uint32_t q0 = t; q0 < U; q0 += 4 * nt) {
+  uint32_t sq[4], bq[4], cq[4], dq[4];
+  K lq[4], hq[4];
+#pragma unroll
+  for (uint32_t u = 0; u < 4; ++u) {
+    const uint32_t q = q0 + u * nt;
+    const bool in = q < U;
+    sq[u] = in ? a.act_slot[q] : kNone32;
+    bq[u] = in ? a.act_bucket[q] : 0u;
+    cq[u] = in ? a.act_cnt[q] : 0u;
+    dq[u] = in ? a.act_dst[q] : 0u;
+    lq[u] = in ? a.act_klo[q] : (K)0;
+    hq[u] = in ? a.act_khi[q] : (K)0;
+  }
+#pragma unroll
+  for (uint32_t u = 0; u < 4; ++u) {
+    const uint32_t s = sq[u];
     if (s == kNone32) continue;
-    uint32_t b = a.act_bucket[q];
-    uint32_t c = a.act_cnt[q];
-    uint32_t dst = a.act_dst[q];
+    const uint32_t b = bq[u], c = cq[u], dst = dq[u];
     a.slot_of_bucket[b] = (uint16_t)s;
     a.slot_dst[s] = dst;
     a.slot_bucket[s] = b;
@@ -1562,8 +1576,8 @@ __global__ void __launch_bounds__(1024) k_plan(PlanArgs<typename KeyTraits<T>::K
     }
     if (s < U_big) {
       Range<K> nr;
-      nr.lo = a.act_klo[q];
-      nr.hi = a.act_khi[q];
+      nr.lo = lq[u];
+      nr.hi = hq[u];
       nr.seg_start = dst;
       nr.seg_end = dst + c;
       nr.base = 0;
@@ -1577,7 +1591,7 @@ __global__ void __launch_bounds__(1024) k_plan(PlanArgs<typename KeyTraits<T>::K
       a.next_ranges[s] = nr;
     } else {
       uint32_t j = s - U_big;
-      const K klo = a.act_klo[q], khi = a.act_khi[q];
+      const K klo = lq[u], khi = hq[u];
       a.small_start[j] = dst;
       a.small_len[j] = c;
       small_max = c > small_max ? c : small_max;
@@ -1586,6 +1600,7 @@ __global__ void __launch_bounds__(1024) k_plan(PlanArgs<typename KeyTraits<T>::K
       if (lcache) sinfo[j] = order_bin_info<K>(c, klo, khi, a.fin_cap);
       kfill[j] = 0;
     }
+  }
   }
 #pragma unroll
   for (int d = 32; d > 0; d >>= 1) {  // one LDS atomic per wave
@@ -1637,25 +1652,44 @@ __global__ void __launch_bounds__(1024) k_plan(PlanArgs<typename KeyTraits<T>::K
     B_next = tot;
   }
 
-  // P7: re-rank live ks
+  // P7: re-rank live ks, four per thread at a time (their loads issued together)
   T* out = reinterpret_cast<T*>(a.out);
-  for (uint32_t k = t; k < a.nk; k += nt) {
-    if (a.k_state[k] != kLive) continue;
-    uint32_t b = a.k_aux[k];
-    uint32_t q = aidx[b] - 1;
-    uint32_t local = a.k_rank[k];
-    uint32_t cls = a.act_cls[q];
-    if (cls == kResolved) {
-      out[k] = KeyTraits<T>::dec(a.act_klo[q]);
-      a.k_state[k] = kDone;
-    } else if (cls == kClsBig) {
-      a.k_rank[k] = a.act_dst[q] + local;
-    } else {
-      uint32_t j = a.act_slot[q] - U_big;
-      a.k_rank[k] = local;
-      a.k_aux[k] = j;
-      a.k_state[k] = kSmallNew;
-      atomicAdd(&kfill[j], 1u);
+  for (uint32_t k0 = t; k0 < a.nk; k0 += 4 * nt) {
+    uint32_t stq[4], bq[4], lq[4];
+#pragma unroll
+    for (uint32_t u = 0; u < 4; ++u) {
+      const uint32_t k = k0 + u * nt;
+      stq[u] = k < a.nk ? a.k_state[k] : kDone;
+      bq[u] = k < a.nk ? a.k_aux[k] : 0u;
+      lq[u] = k < a.nk ? a.k_rank[k] : 0u;
+    }
+    uint32_t qq[4], cq[4], dq[4], sq[4];
+    K kq[4];
+#pragma unroll
+    for (uint32_t u = 0; u < 4; ++u) {
+      const bool live = stq[u] == kLive;
+      qq[u] = live ? aidx[bq[u]] - 1 : 0u;
+      cq[u] = live ? a.act_cls[qq[u]] : kResolved;
+      dq[u] = live ? a.act_dst[qq[u]] : 0u;
+      sq[u] = live ? a.act_slot[qq[u]] : 0u;
+      kq[u] = live ? a.act_klo[qq[u]] : (K)0;
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < 4; ++u) {
+      if (stq[u] != kLive) continue;
+      const uint32_t k = k0 + u * nt;
+      if (cq[u] == kResolved) {
+        out[k] = KeyTraits<T>::dec(kq[u]);
+        a.k_state[k] = kDone;
+      } else if (cq[u] == kClsBig) {
+        a.k_rank[k] = dq[u] + lq[u];
+      } else {
+        uint32_t j = sq[u] - U_big;
+        a.k_rank[k] = lq[u];
+        a.k_aux[k] = j;
+        a.k_state[k] = kSmallNew;
+        atomicAdd(&kfill[j], 1u);
+      }
     }
   }
   __threadfence_block();
@@ -1673,7 +1707,9 @@ __global__ void __launch_bounds__(1024) k_plan(PlanArgs<typename KeyTraits<T>::K
     uint32_t s = 0;
     for (uint32_t j = j0; j < j1; ++j) {
       s += kfill[j];
-      atomicAdd(&ord_cnt[order_bin_of(binfo(j), kfill[j], waves)], 1u);
+      // few size classes, many segments: one LDS atomic per wave for a class
+      // shared by many lanes
+      lds_count(ord_cnt, order_bin_of(binfo(j), kfill[j], waves));
     }
     uint32_t tot;
     uint32_t off = block_excl_scan_u32(s, scratch, &tot);
@@ -1701,15 +1737,25 @@ __global__ void __launch_bounds__(1024) k_plan(PlanArgs<typename KeyTraits<T>::K
     }
   }
   uint32_t live = 0;
-  for (uint32_t k = t; k < a.nk; k += nt) {
-    uint32_t st = a.k_state[k];
-    if (st == kSmallNew) {
-      uint32_t j = a.k_aux[k];
-      uint32_t pos = atomicAdd(&kfill[j], 1u);
-      a.klist[kstart(j) + pos] = k;
-      a.k_state[k] = kSmall;
+  for (uint32_t k0 = t; k0 < a.nk; k0 += 4 * nt) {  // four ks per thread at a time
+    uint32_t stq[4], jq[4];
+#pragma unroll
+    for (uint32_t u = 0; u < 4; ++u) {
+      const uint32_t k = k0 + u * nt;
+      stq[u] = k < a.nk ? a.k_state[k] : kDone;
+      jq[u] = stq[u] == kSmallNew ? a.k_aux[k] : 0u;
     }
-    if (st == kLive) live++;
+#pragma unroll
+    for (uint32_t u = 0; u < 4; ++u) {
+      const uint32_t k = k0 + u * nt;
+      if (stq[u] == kSmallNew) {
+        const uint32_t j = jq[u];
+        const uint32_t pos = atomicAdd(&kfill[j], 1u);
+        a.klist[kstart(j) + pos] = k;
+        a.k_state[k] = kSmall;
+      }
+      if (stq[u] == kLive) live++;
+    }
   }
   uint32_t live_tot;
   block_excl_scan_u32(live, scratch, &live_tot);  // (its barriers also publish ord_cnt)
@@ -1717,7 +1763,7 @@ __global__ void __launch_bounds__(1024) k_plan(PlanArgs<typename KeyTraits<T>::K
   // workgroups start early instead of trailing the launch
   for (uint32_t j = t; j < U_small; j += nt) {
     const uint32_t nks = kstart(j + 1) - kstart(j);
-    a.small_order[atomicAdd(&ord_cnt[order_bin_of(binfo(j), nks, waves)], 1u)] = j;
+    a.small_order[lds_claim(ord_cnt, order_bin_of(binfo(j), nks, waves), true)] = j;
   }
 
   // P9: next-pass tiling and first range of every tile
