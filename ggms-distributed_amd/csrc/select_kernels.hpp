@@ -2537,7 +2537,7 @@ __device__ K radix_select_lds(const K* __restrict__ s, uint32_t len, K klo, K kh
 }
 
 struct SelectLds {
-  uint32_t hist[kSelectMaxKs * 256];
+  alignas(16) uint32_t hist[kSelectMaxKs * 256];
   uint32_t kid[kSelectMaxKs];   // k index, sorted by rank
   uint32_t rank[kSelectMaxKs];  // 1-based rank inside the segment, then inside the prefix
   uint32_t grp[kSelectMaxKs];   // histogram of the k's prefix this round
@@ -2546,21 +2546,124 @@ struct SelectLds {
   uint64_t gpre[kSelectMaxKs];  // distinct prefixes of this round
 };
 
+// Digits of a radix select over the low `vary` bits of a segment's keys (all
+// keys share the bits above): the first digit is up to kSelectFirstBits wide
+// (every k starts from the same prefix, so one histogram of 2^w bins, the
+// whole of SelectLds::hist), the later ones up to 8 bits (one 256-bin
+// histogram per distinct prefix the ks have reached).  A first digit of 11
+// bits resolves most segments of a bucketing pass in one sweep: their
+// intervals span a few thousand key values.
+constexpr uint32_t kSelectFirstBits = 11;
+static_assert((1u << kSelectFirstBits) <= kSelectMaxKs * 256, "first digit histogram");
+
+__host__ __device__ __forceinline__ void select_first_digit(uint32_t vary, uint32_t& shift, uint32_t& width) {
+  width = vary < kSelectFirstBits ? vary : kSelectFirstBits;
+  shift = vary - width;
+}
+
+// Clears the first digit's histogram: all of SelectLds::hist (the scan reads
+// every bin whatever the digit's width; clearing only the bins a narrow digit
+// uses and scanning only those measured slower, the extra paths costing the
+// finish kernel its two workgroups per CU or more spills).  Whole workgroup;
+// a barrier must follow before the counting.
+__device__ __forceinline__ void select_zero_first(SelectLds& L) {
+  uint4* h = reinterpret_cast<uint4*>(L.hist);
+  for (uint32_t i = threadIdx.x; i < kSelectMaxKs * 64; i += blockDim.x) h[i] = make_uint4(0, 0, 0, 0);
+}
+
+// Count key (any value of the segment) into the first digit's histogram.
+template <class K>
+__device__ __forceinline__ void select_count_first(uint32_t* hist, K key, uint32_t shift, uint32_t width, bool active) {
+  lds_count(hist, (uint32_t)(key >> shift) & ((1u << width) - 1u), active);
+}
+
 // On entry kid/rank hold q (<= kSelectMaxKs) ks sorted by rank; on exit pre[j]
 // is the key of rank rank[j] among s[0, len).  Whole workgroup.  s is the
 // segment in LDS, or in global memory for a segment too wide and too large
-// for the LDS layouts (each digit then reads it from the caches again).
+// for the LDS layouts (each digit then reads it from the caches again).  With
+// `counted` the caller has already built the first digit's histogram in
+// L.hist (zeroed, then select_count_first for every key, then a barrier),
+// typically while loading the segment; s is then read only for the later
+// digits (not at all when the first digit covers every varying bit).
 template <class K>
-__device__ void radix_select_multi(const K* __restrict__ s, uint32_t len, K klo, K khi, uint32_t q, SelectLds& L) {
+__device__ void radix_select_multi(const K* __restrict__ s, uint32_t len, K klo, K khi, uint32_t q, SelectLds& L,
+                                   bool counted = false) {
   constexpr uint32_t kb = sizeof(K) * 8;
   const uint32_t vary = bitlen64((uint64_t)(klo ^ khi));
   const K base = vary >= kb ? (K)0 : (K)(klo & (K)(~(K)0 << vary));
-  const uint32_t top = (vary + 7) & ~7u;
   const uint32_t wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
   if (threadIdx.x < q) L.pre[threadIdx.x] = base;
+  // every key once, four reads in flight per thread (s may be LDS or global)
+  auto sweep = [&](auto&& f) {
+    const uint32_t nt = blockDim.x;
+    uint32_t i = threadIdx.x;
+    for (; i + 3 * nt < len; i += 4 * nt) {
+      const K k0 = s[i], k1 = s[i + nt], k2 = s[i + 2 * nt], k3 = s[i + 3 * nt];
+      f(k0);
+      f(k1);
+      f(k2);
+      f(k3);
+    }
+    for (; i < len; i += nt) f(s[i]);
+  };
+  uint32_t sh1, w1;
+  select_first_digit(vary, sh1, w1);
+  if (w1 > 0) {
+    if (!counted) {
+      select_zero_first(L);
+      __syncthreads();
+      sweep([&](K key) { select_count_first<K>(L.hist, key, sh1, w1, true); });
+    }
+    __syncthreads();
+    // each k scans the histogram with one wave, 32 bins a lane read as eight
+    // 16-byte words (one LDS round trip); the lane where the rank falls walks
+    // its bins in registers, half at a time
+    const uint32_t l = threadIdx.x & 63;
+    const uint4* h4 = reinterpret_cast<const uint4*>(L.hist) + 8 * l;
+    for (uint32_t j = wv; j < q; j += nw) {
+      const uint32_t want = L.rank[j];
+      uint32_t lo = 0, hi = 0;
+#pragma unroll
+      for (int v = 0; v < 8; ++v) {
+        const uint4 x = h4[v];
+        (v < 4 ? lo : hi) += x.x + x.y + x.z + x.w;
+      }
+      uint32_t cum = wave_incl_scan_u32(lo + hi) - (lo + hi);
+      if (cum < want && cum + lo + hi >= want) {  // exactly one lane
+        const bool second = cum + lo < want;
+        uint32_t c[16];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const uint4 x = h4[(second ? 4 : 0) + v];
+          c[4 * v] = x.x;
+          c[4 * v + 1] = x.y;
+          c[4 * v + 2] = x.z;
+          c[4 * v + 3] = x.w;
+        }
+        if (second) cum += lo;
+        uint32_t b = 32 * l + (second ? 16u : 0u), rr = 0;
+        bool done = false;
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+          if (!done && cum + c[e] >= want) {
+            b += (uint32_t)e;
+            rr = want - cum;
+            done = true;
+          }
+          cum += c[e];
+        }
+        L.pre[j] = (K)((K)L.pre[j] | ((K)b << sh1));
+        L.rank[j] = rr;
+      }
+    }
+  }
   __syncthreads();
-  for (int shift = (int)top - 8; shift >= 0; shift -= 8) {
-    const K hmask = (uint32_t)shift + 8 >= kb ? (K)0 : (K)(~(K)0 << (shift + 8));
+  for (uint32_t rem = sh1; rem > 0;) {
+    const uint32_t w = rem < 8 ? rem : 8u;
+    const uint32_t shift = rem - w;
+    rem = shift;
+    const uint32_t dmask = (1u << w) - 1u;
+    const K hmask = shift + w >= kb ? (K)0 : (K)(~(K)0 << (shift + w));
     if (threadIdx.x == 0) {
       uint32_t ng = 0;
       for (uint32_t j = 0; j < q; ++j) {
@@ -2577,21 +2680,8 @@ __device__ void radix_select_multi(const K* __restrict__ s, uint32_t len, K klo,
     for (uint32_t i = 0; i < kSelectMaxKs; ++i) gp[i] = (K)L.gpre[i];
     for (uint32_t i = threadIdx.x; i < ng * 256; i += blockDim.x) L.hist[i] = 0;
     __syncthreads();
-    // every key once, four reads in flight per thread (s may be LDS or global)
-    auto sweep = [&](auto&& f) {
-      const uint32_t nt = blockDim.x;
-      uint32_t i = threadIdx.x;
-      for (; i + 3 * nt < len; i += 4 * nt) {
-        const K k0 = s[i], k1 = s[i + nt], k2 = s[i + 2 * nt], k3 = s[i + 3 * nt];
-        f(k0);
-        f(k1);
-        f(k2);
-        f(k3);
-      }
-      for (; i < len; i += nt) f(s[i]);
-    };
     if (ng == 1) {  // every k shares one prefix: the common case
-      sweep([&](K key) { lds_count(L.hist, (uint32_t)(key >> shift) & 255u, (K)(key & hmask) == gp[0]); });
+      sweep([&](K key) { lds_count(L.hist, (uint32_t)(key >> shift) & dmask, (K)(key & hmask) == gp[0]); });
     } else {
       sweep([&](K key) {
         const K km = (K)(key & hmask);
@@ -2599,7 +2689,7 @@ __device__ void radix_select_multi(const K* __restrict__ s, uint32_t len, K klo,
 #pragma unroll
         for (uint32_t p = 0; p < kSelectMaxKs; ++p)
           if (p < ng && gp[p] == km) g = p;
-        lds_count(L.hist, g * 256 + ((uint32_t)(key >> shift) & 255u), g < ng);
+        lds_count(L.hist, g * 256 + ((uint32_t)(key >> shift) & dmask), g < ng);
       });
     }
     __syncthreads();
@@ -2622,17 +2712,17 @@ __device__ void radix_select_multi(const K* __restrict__ s, uint32_t len, K klo,
   }
 }
 
-// Loads buf[st, st + len) into s[0, len) with 16-byte loads, up to kSegLoadVecs
-// per lane in flight (a segment of the finish capacity is one round trip),
-// storing key - base as type C (the key itself when C is K and base is 0).
-// The vectors are aligned down/up to 16 bytes; the keys outside the segment
-// they carry are dropped.  Every buffer this reads is allocated in 256-byte
-// multiples from a 256-byte aligned base, so the rounded vectors stay inside it.
+// Reads buf[st, st + len) with 16-byte loads, up to kSegLoadVecs per lane in
+// flight (a segment of the finish capacity is one round trip), and calls
+// f(i, key, in) for every key of every vector: i its index in the segment, in
+// whether it belongs to the segment (the vectors are aligned down/up to 16
+// bytes; i wraps above len for the head keys before st).  Every buffer this
+// reads is allocated in 256-byte multiples from a 256-byte aligned base, so
+// the rounded vectors stay inside it.
 constexpr int kSegLoadVecs = 4;
 
-template <class K, class C>
-__device__ __forceinline__ void load_segment_lds(C* __restrict__ s, const K* __restrict__ buf, uint32_t st, uint32_t len,
-                                                 K base) {
+template <class K, class F>
+__device__ __forceinline__ void load_segment(const K* __restrict__ buf, uint32_t st, uint32_t len, F&& f) {
   constexpr uint32_t N = 16 / sizeof(K);
   const uint32_t head = st & (N - 1);
   const uint4* src = reinterpret_cast<const uint4*>(buf + (st - head));
@@ -2642,8 +2732,8 @@ __device__ __forceinline__ void load_segment_lds(C* __restrict__ s, const K* __r
     __builtin_memcpy(e, &v, 16);
 #pragma unroll
     for (uint32_t q = 0; q < N; ++q) {
-      uint32_t idx = i * N + q - head;  // wraps above len for the dropped head keys
-      if (idx < len) s[idx] = (C)(e[q] - base);
+      const uint32_t idx = i * N + q - head;
+      f(idx, e[q], idx < len);
     }
   };
   uint4 v[kSegLoadVecs];
@@ -2662,13 +2752,14 @@ __device__ __forceinline__ void load_segment_lds(C* __restrict__ s, const K* __r
 
 // Answers the ks [q0, q1) of one segment held in LDS as type C, C value = key - base.
 // s is writable LDS, except for the global-memory case, which passes at most
-// kSelectMaxKs ks and so never reaches the sort.
+// kSelectMaxKs ks and so never reaches the sort.  counted: see
+// radix_select_multi (at most kSelectMaxKs ks).
 template <class T, class C>
 __device__ __forceinline__ void finish_segment(const C* __restrict__ s, uint32_t len, C clo, C chi,
                                                typename KeyTraits<T>::K base, uint32_t q0, uint32_t q1,
                                                uint32_t kk, uint32_t kr, const uint32_t* __restrict__ klist,
                                                uint32_t* __restrict__ k_rank, uint32_t* __restrict__ k_state,
-                                               T* __restrict__ out, SelectLds& L) {
+                                               T* __restrict__ out, SelectLds& L, bool counted = false) {
   using K = typename KeyTraits<T>::K;
   const uint32_t q = q1 - q0;
   if (q <= kSelectMaxKs) {
@@ -2690,7 +2781,7 @@ __device__ __forceinline__ void finish_segment(const C* __restrict__ s, uint32_t
       }
     }
     __syncthreads();
-    radix_select_multi<C>(s, len, clo, chi, q, L);
+    radix_select_multi<C>(s, len, clo, chi, q, L, counted);
     if (threadIdx.x < q) {
       const uint32_t k = L.kid[threadIdx.x];
       out[k] = KeyTraits<T>::dec((K)(base + (K)(C)L.pre[threadIdx.x]));
@@ -2748,15 +2839,28 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8)
       kk = klist[q0 + threadIdx.x];
       kr = k_rank[kk];
     }
-    if (len > fin_cap / 2) {  // narrow interval: half-width offsets from klo
-      H* sh = reinterpret_cast<H*>(smem);
-      load_segment_lds<K, H>(sh, buf, st, len, klo);
-      finish_segment<T, H>(sh, len, (H)0, (H)(khi - klo), klo, q0, q1, kk, kr, klist, k_rank, k_state, out, L);
-    } else {
-      K* sk = reinterpret_cast<K*>(smem);
-      load_segment_lds<K, K>(sk, buf, st, len, (K)0);
-      finish_segment<T, K>(sk, len, klo, khi, (K)0, q0, q1, kk, kr, klist, k_rank, k_state, out, L);
-    }
+    // The segment is loaded into LDS as C values (key - base).  With at most
+    // kSelectMaxKs ks the load also counts the radix select's first digit,
+    // and stores the keys only when later digits need them.
+    auto run = [&](auto* sv, auto clo, auto chi, K base) {
+      using C = typename std::remove_pointer<decltype(sv)>::type;
+      const bool sel = q1 - q0 <= kSelectMaxKs;
+      uint32_t sh1, w1;
+      select_first_digit(bitlen64((uint64_t)(clo ^ chi)), sh1, w1);
+      const bool keep = !sel || sh1 > 0;
+      if (sel) {
+        select_zero_first(L);
+        __syncthreads();
+      }
+      load_segment<K>(buf, st, len, [&](uint32_t i, K key, bool in) {
+        const C c = (C)(key - base);
+        if (in && keep) sv[i] = c;
+        if (sel) select_count_first<C>(L.hist, c, sh1, w1, in);
+      });
+      finish_segment<T, C>(sv, len, clo, chi, base, q0, q1, kk, kr, klist, k_rank, k_state, out, L, sel);
+    };
+    if (len > fin_cap / 2) run(reinterpret_cast<H*>(smem), (H)0, (H)(khi - klo), klo);  // narrow interval
+    else run(reinterpret_cast<K*>(smem), klo, khi, (K)0);
     __syncthreads();
   }
   // segments of at most kFinishWaveLen keys and kFinishWaveKs ks: one wave
