@@ -183,7 +183,9 @@ __device__ __forceinline__ void lds_count(uint32_t* hist, uint32_t b, bool activ
   uint64_t m0 = wave_ballot(same);
   uint32_t c0 = (uint32_t)__builtin_popcountll(m0);
   if (c0 >= kPeelMin) {
-    if (lane_id() == first) atomicAdd(&hist[b0], c0);
+    // b == b0 in lane `first`; the lane's own b keeps the address divergent
+    // (see lds_claim)
+    if (lane_id() == first) atomicAdd(&hist[b], c0);
     if (active && !same) atomicAdd(&hist[b], 1u);
   } else if (active) {
     atomicAdd(&hist[b], 1u);
@@ -202,7 +204,11 @@ __device__ __forceinline__ uint32_t lds_claim(uint32_t* cursor, uint32_t s, bool
   uint32_t pos = 0;
   if (c0 >= kPeelMin) {
     uint32_t base0 = 0;
-    if (lane_id() == first) base0 = atomicAdd(&cursor[s0], c0);
+    // s == s0 in lane `first`.  Through the lane's own s the address is
+    // divergent, so the compiler does not wrap this one-lane atomic in the
+    // whole-wave reduction it builds for uniform addresses (count the active
+    // lanes, elect one, broadcast), which costs ~5 VALU and ~5 SALU each time.
+    if (lane_id() == first) base0 = atomicAdd(&cursor[s], c0);
     base0 = readlane_u32(base0, first);
     if (same) pos = base0 + mask_below(m0);
     else if (active) pos = atomicAdd(&cursor[s], 1u);

@@ -828,6 +828,9 @@ template <class K> struct Pass0Map {
   uint32_t* psh;    // 64-bit keys
   uint64_t* pc;     // 32-bit keys
   uint32_t Rn = 0;  // ranges
+  // (Per-field arrays rather than one record per range: random ranges across
+  // a wave then spread over all LDS banks; 16-byte records put every lane's
+  // first word on a quarter of them and measured slower in the pass-1 kernels.)
 
   // LDS bytes of the parameter arrays for R ranges (16 bytes per range for
   // 32-bit keys, 20 for 64-bit keys), rounded to 16
@@ -892,11 +895,6 @@ template <class K> struct Pass0Map {
       return pbase[r] + h + id;
     }
   }
-  // Range of a key in a level-1 leaf entry: r0 (the bits above S1; the flag
-  // bit is 0 for a leaf) plus one when the key's in-cell offset passes the
-  // stored start offset.  Splicing the key's low bits under the entry's high
-  // bits turns that test into one compare of whole words.  Meaningless for
-  // non-leaf entries, which the callers overwrite.
   // Range of a key in a level-1 leaf entry: r0 (the bits above S1; the flag
   // bit is 0 for a leaf) plus one when the key's in-cell offset passes the
   // stored start offset.  Splicing the in-cell offset under the entry's high
@@ -1908,9 +1906,14 @@ __global__ void __launch_bounds__(1024) k_compact0(const T* __restrict__ x, cons
           tot += (uint32_t)__builtin_popcountll(m[j]);
         }
         if (tot) {  // one run claimed for the wave's survivors of all N loads
-          uint32_t first = (uint32_t)__builtin_ctzll(wave_ballot(true));
+          const uint64_t ex = wave_ballot(true);
+          const uint32_t first = (uint32_t)__builtin_ctzll(ex);
+          // mask_below(ex) is 0 in lane `first`; as a divergent offset it
+          // keeps the compiler from wrapping this one-lane atomic in a
+          // whole-wave reduction (see lds_claim)
+          const uint32_t z = mask_below(ex);
           uint32_t base = 0;
-          if (lane_id() == first) base = atomicAdd(&cursor, tot);
+          if (lane_id() == first) base = atomicAdd(&cursor + z, tot);
           // wave-uniform run pointer: each store is scalar base + 32-bit lane offset
           K* run = out + readlane_u32(base, first);
 #pragma unroll
