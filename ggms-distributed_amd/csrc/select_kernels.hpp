@@ -2585,9 +2585,10 @@ __device__ __forceinline__ void select_count_first(uint32_t* hist, K key, uint32
 // segment in LDS, or in global memory for a segment too wide and too large
 // for the LDS layouts (each digit then reads it from the caches again).  With
 // `counted` the caller has already built the first digit's histogram in
-// L.hist (zeroed, then select_count_first for every key, then a barrier),
-// typically while loading the segment; s is then read only for the later
-// digits (not at all when the first digit covers every varying bit).
+// L.hist (zeroed, then select_count_first for every key, then a barrier --
+// the one after its writes of kid/rank serves), typically while loading the
+// segment; s is then read only for the later digits (not at all when the
+// first digit covers every varying bit).
 template <class K>
 __device__ void radix_select_multi(const K* __restrict__ s, uint32_t len, K klo, K khi, uint32_t q, SelectLds& L,
                                    bool counted = false) {
@@ -2595,7 +2596,6 @@ __device__ void radix_select_multi(const K* __restrict__ s, uint32_t len, K klo,
   const uint32_t vary = bitlen64((uint64_t)(klo ^ khi));
   const K base = vary >= kb ? (K)0 : (K)(klo & (K)(~(K)0 << vary));
   const uint32_t wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  if (threadIdx.x < q) L.pre[threadIdx.x] = base;
   // every key once, four reads in flight per thread (s may be LDS or global)
   auto sweep = [&](auto&& f) {
     const uint32_t nt = blockDim.x;
@@ -2611,13 +2611,15 @@ __device__ void radix_select_multi(const K* __restrict__ s, uint32_t len, K klo,
   };
   uint32_t sh1, w1;
   select_first_digit(vary, sh1, w1);
-  if (w1 > 0) {
+  if (w1 == 0) {  // every key equal
+    if (threadIdx.x < q) L.pre[threadIdx.x] = base;
+  } else {
     if (!counted) {
       select_zero_first(L);
       __syncthreads();
       sweep([&](K key) { select_count_first<K>(L.hist, key, sh1, w1, true); });
+      __syncthreads();
     }
-    __syncthreads();
     // each k scans the histogram with one wave, 32 bins a lane read as eight
     // 16-byte words (one LDS round trip); the lane where the rank falls walks
     // its bins in registers, half at a time
@@ -2655,7 +2657,7 @@ __device__ void radix_select_multi(const K* __restrict__ s, uint32_t len, K klo,
           }
           cum += c[e];
         }
-        L.pre[j] = (K)((K)L.pre[j] | ((K)b << sh1));
+        L.pre[j] = (K)(base | ((K)b << sh1));
         L.rank[j] = rr;
       }
     }
@@ -2765,7 +2767,18 @@ __device__ __forceinline__ void finish_segment(const C* __restrict__ s, uint32_t
                                                T* __restrict__ out, SelectLds& L, bool counted = false) {
   using K = typename KeyTraits<T>::K;
   const uint32_t q = q1 - q0;
-  if (q <= kSelectMaxKs) {
+  if (q == 1) {
+    if (threadIdx.x == 0) {
+      L.kid[0] = kk;
+      L.rank[0] = kr;
+    }
+    __syncthreads();
+    radix_select_multi<C>(s, len, clo, chi, 1, L, counted);
+    if (threadIdx.x == 0) {
+      out[kk] = KeyTraits<T>::dec((K)(base + (K)(C)L.pre[0]));
+      k_state[kk] = kDone;
+    }
+  } else if (q <= kSelectMaxKs) {
     if (threadIdx.x < q) {
       L.kid[threadIdx.x] = kk;
       L.rank[threadIdx.x] = kr;
@@ -2826,6 +2839,10 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8)
   // finish order: [whole-workgroup segments | U_wave one-wave segments | U_wide (k_finish_wide)]
   const uint32_t U_small = ctl->U_small, U_wave = ctl->U_wave;
   const uint32_t U_grp = U_small - U_wave - ctl->U_wide;
+  // the first digit's histogram starts cleared; each segment that counts
+  // into it clears it again once done
+  select_zero_first(L);
+  __syncthreads();
   for (uint32_t jj = blockIdx.x; jj < U_grp; jj += gridDim.x) {
     const uint32_t j = small_order ? small_order[jj] : jj;
     const uint32_t st = small_start[j], len = small_len[j];
@@ -2851,16 +2868,13 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8)
       uint32_t sh1, w1;
       select_first_digit(bitlen64((uint64_t)(clo ^ chi)), sh1, w1);
       const bool keep = !sel || sh1 > 0;
-      if (sel) {
-        select_zero_first(L);
-        __syncthreads();
-      }
       load_segment<K>(buf, st, len, [&](uint32_t i, K key, bool in) {
         const C c = (C)(key - base);
         if (in && keep) sv[i] = c;
         if (sel) select_count_first<C>(L.hist, c, sh1, w1, in);
       });
       finish_segment<T, C>(sv, len, clo, chi, base, q0, q1, kk, kr, klist, k_rank, k_state, out, L, sel);
+      if (sel) select_zero_first(L);  // for the next segment (the barrier below orders it)
     };
     if (len > fin_cap / 2) run(reinterpret_cast<H*>(smem), (H)0, (H)(khi - klo), klo);  // narrow interval
     else run(reinterpret_cast<K*>(smem), klo, khi, (K)0);

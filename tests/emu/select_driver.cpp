@@ -48,6 +48,7 @@ template <class K> int run(uint64_t seed, uint32_t nt, bool counted) {
         ggms::select_zero_first(L);
         __syncthreads();
         for (uint32_t i = t; i < len; i += nt) ggms::select_count_first<K>(L.hist, a[i], sh1, w1, true);
+        __syncthreads();
         if (sh1 == 0) s = nullptr;
       }
       ggms::radix_select_multi<K>(s, len, klo, khi, q, L, counted);
