@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -168,7 +169,8 @@ struct ggms_ctx {
   void* pinned = nullptr;  // host staging (ctl blocks, ks, outputs)
   void* pinned_dev = nullptr;  // the same memory as the device addresses it
   size_t pinned_bytes = 0;
-  hipEvent_t plan_ev = nullptr;  // recorded after each pass's plan kernel
+  hipEvent_t plan_ev = nullptr;  // kept for callers of older layouts (unused by the pass loop)
+  uint32_t mirror_seq = 0;       // last sequence number a plan kernel was asked to publish
   ggms_stats stats{};
   std::vector<hipEvent_t> events;  // timing events (opts.profile)
   std::mutex mu;
@@ -695,6 +697,23 @@ static ggms_status fixed_status(const uint32_t* f) {
   return GGMS_OK;
 }
 
+// Waits until the plan kernel has published sequence number `want` (after
+// its mirror of the control blocks, with a system-scope fence between).  Every
+// 65536 polls it asks whether the stream has drained: a drained stream without
+// the word means the plan never published (a device failure), not a wait.
+static ggms_status wait_mirror(volatile uint32_t* seq, uint32_t want, hipStream_t st) {
+  for (uint64_t spins = 1; *seq != want; ++spins) {
+    if ((spins & 0xFFFF) == 0) {
+      const hipError_t q = hipStreamQuery(st);
+      if (q == hipSuccess) return *seq == want ? GGMS_OK : GGMS_EDEVICE;
+      if (q != hipErrorNotReady) return GGMS_EDEVICE;
+    }
+    __builtin_ia32_pause();
+  }
+  std::atomic_thread_fence(std::memory_order_acquire);
+  return GGMS_OK;
+}
+
 template <class T>
 static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint64_t n64, const uint32_t* ks,
                               bool ks_on_device, uint32_t nk, void* out, bool out_on_device, const ggms_opts& o) {
@@ -754,6 +773,8 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
   uint32_t* dflag = (uint32_t*)(Hd + 6 * ctlb);
   volatile uint32_t* hbad = hflag + 1;
   uint32_t* dbad = dflag + 1;
+  volatile uint32_t* hseq = hflag + 3;  // the plan's mirror sequence word (written after the mirror)
+  uint32_t* dseq = dflag + 3;
   char* hks = H + 6 * ctlb + 256;
   const uint32_t* d_ks = ks_on_device ? ks : (const uint32_t*)(Hd + 6 * ctlb + 256);
   char* hout = hks + ks_bytes;
@@ -910,6 +931,9 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
       pl.next_params = (pass == 0 && !c.seg_first) ? (MapParam<K>*)P(bParams1) : nullptr;
       pl.rank_words = (pass == 0 && !c.seg_first) ? (uint64_t*)P(bRank0) : nullptr;
       pl.mirror = dmirror;
+      pl.mirror_seq = dseq;
+      if (++ctx->mirror_seq == 0) ++ctx->mirror_seq;
+      pl.seq = ctx->mirror_seq;
       pl.nan_out = nullptr;
       plan_groups<K>(pl, L, W, pass == 0 && c.seg_first);
       size_t lds_plan = (size_t)B_bound * 8 + (size_t)rank_word_count(B_bound) * 8;
@@ -919,7 +943,6 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
       k_plan<T><<<1, 1024, lds_plan, st>>>(pl);
       prof.end();
       HIP_TRY(hipGetLastError());
-      HIP_TRY(hipEventRecord(ctx->plan_ev, st));
       // compaction
       if (pass == 0 && c.seg_first) {
         // every k-bucket into its own segment (the finish reads segments)
@@ -958,8 +981,12 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
       }
       HIP_TRY(hipGetLastError());
       // the next pass (and this pass's finish) is planned from the control
-      // blocks the plan kernel wrote to host memory, while the compaction runs
-      HIP_TRY(hipEventSynchronize(ctx->plan_ev));
+      // blocks the plan kernel wrote to host memory, while the compaction
+      // runs.  The host polls the sequence word the plan writes after them:
+      // an event recorded between the plan and the compaction would make the
+      // GPU drain and publish at that point (~6 us idle before each compaction).
+      s = wait_mirror(hseq, pl.seq, st);
+      if (s) return s;
       const PassCtl h = hmirror[0];
       const PassCtl hn = hmirror[1];
       // the iterative first pass hands every k-bucket to pass 1

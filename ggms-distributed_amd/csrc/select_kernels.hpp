@@ -1382,6 +1382,8 @@ template <class K> struct PlanArgs {
   MapParam<K>* next_params;  // packed map parameters of next_ranges (or null)
   uint64_t* rank_words;      // slot rank words of this pass's buckets (or null)
   PassCtl* mirror;           // host-visible copy of ctl and ctl_next, written last (or null)
+  uint32_t* mirror_seq;      // with a mirror: seq is written here after it (host-visible)
+  uint32_t seq;
   uint32_t* nan_out;         // the first pass writes its NaN count here (or null)
   // grouped compaction (or null): slot s belongs to group s / kGroupSlots; the
   // slots of a group are contiguous in the output, so a group owns one span
@@ -1811,6 +1813,8 @@ __global__ void __launch_bounds__(1024) k_plan(PlanArgs<typename KeyTraits<T>::K
     if (a.mirror) {
       a.mirror[0] = *cc;
       a.mirror[1] = *cn;
+      __threadfence_system();  // the mirror reaches host memory before the sequence word
+      *reinterpret_cast<volatile uint32_t*>(a.mirror_seq) = a.seq;
     }
   }
 }
