@@ -918,44 +918,46 @@ template <class K> struct Pass0Map {
       return (uint32_t)(key >> G::S1);
     }
   }
-  // The rare lookups: a level-2 table, a multi-start (sub)cell or a key
-  // outside the window (binary search over the range starts).
-  template <bool WIN> __device__ __forceinline__ uint32_t range_slow(K key, K e, bool out, uint32_t r) const {
-    uint32_t lo = 0, hi = 0;
-    bool search = out;
-    if (out) {  // the few ranges outside the window
+  // The rarer lookups, each behind its own wave-uniform test: a level-2 table
+  // (one more LDS read), then a binary search over the range starts for a
+  // multi-start (sub)cell or a key outside the window.  Split in two, a wave
+  // whose keys need level-2 tables but no search (multi-modal inputs: a few
+  // dense cells hold several starts) skips the search code entirely.
+  // level2: the sub-cell entry of key, whose level-1 entry e points to a
+  // table; r becomes its range when that entry is a leaf.
+  template <bool WIN> __device__ __forceinline__ K level2(K key, K e, uint32_t& r) const {
+    uint32_t sub;
+    K soff;
+    if constexpr (WIN) {
+      const K d = (K)(key - wbase);
+      sub = (uint32_t)(d >> (wsh - 6)) & 63u;
+      soff = d & (K)(((K)1 << (wsh - 6)) - 1);
+    } else {
+      sub = (uint32_t)(key >> G::S2) & 63u;
+      soff = key & G::M2;
+    }
+    const K e2 = lut[G::L1CELLS + (((uint32_t)(e & (G::TABLE - 1))) << 6) + sub];
+    r = (uint32_t)((e2 >> G::S2) & 0x1FFFF) + (soff > (e2 & G::M2) ? 1u : 0u);
+    return e2;
+  }
+  // search: the range of key by binary search over the candidates a search
+  // entry e names, or over the ranges below / above the window for a key
+  // outside it
+  __device__ __forceinline__ uint32_t search(K key, K e, bool out) const {
+    uint32_t lo, hi;
+    if (out) {
       const bool below = key < wbase;
       lo = below ? 0u : r_above;
       hi = below ? r_below : (uint32_t)(Rn - 1);
     } else {
-      if (e & G::TABLE) {
-        uint32_t sub;
-        K soff;
-        if constexpr (WIN) {
-          const K d = (K)(key - wbase);
-          sub = (uint32_t)(d >> (wsh - 6)) & 63u;
-          soff = d & (K)(((K)1 << (wsh - 6)) - 1);
-        } else {
-          sub = (uint32_t)(key >> G::S2) & 63u;
-          soff = key & G::M2;
-        }
-        e = lut[G::L1CELLS + (((uint32_t)(e & (G::TABLE - 1))) << 6) + sub];
-        r = (uint32_t)((e >> G::S2) & 0x1FFFF) + (soff > (e & G::M2) ? 1u : 0u);
-      }
-      if (e & G::FLAG) {
-        search = true;
-        lo = (uint32_t)(e & 0x7FF);
-        hi = (uint32_t)((e >> 11) & 0x7FF);
-      }
+      lo = (uint32_t)(e & 0x7FF);
+      hi = (uint32_t)((e >> 11) & 0x7FF);
     }
-    if (search) {
-      while (lo < hi) {
-        uint32_t mid = (lo + hi + 1) >> 1;
-        if (plo[mid] <= key) lo = mid; else hi = mid - 1;
-      }
-      r = lo;
+    while (lo < hi) {
+      uint32_t mid = (lo + hi + 1) >> 1;
+      if (plo[mid] <= key) lo = mid; else hi = mid - 1;
     }
-    return r;
+    return lo;
   }
   template <bool WIN = false> __device__ __forceinline__ uint32_t range_of(K key) const {
     K off;
@@ -964,7 +966,11 @@ template <class K> struct Pass0Map {
     uint32_t r = leaf_range(e, off);
     bool nl = (e & G::FLAG) != 0 || out;
     if (wave_ballot(nl)) {
-      if (nl) r = range_slow<WIN>(key, e, out, r);
+      if (!out && (e & G::TABLE)) e = level2<WIN>(key, e, r);
+      const bool ns = out || (e & G::FLAG) != 0;
+      if (wave_ballot(ns)) {
+        if (ns) r = search(key, e, out);
+      }
     }
     return r;
   }
@@ -987,9 +993,17 @@ template <class K> struct Pass0Map {
       nl |= (e[j] & G::FLAG) != 0 || out[j];
     }
     if (wave_ballot(nl)) {
+      bool ns = false;
 #pragma unroll
-      for (int j = 0; j < N; ++j)
-        if ((e[j] & G::FLAG) || out[j]) r[j] = range_slow<WIN>(key[j], e[j], out[j], r[j]);
+      for (int j = 0; j < N; ++j) {
+        if (!out[j] && (e[j] & G::TABLE)) e[j] = level2<WIN>(key[j], e[j], r[j]);
+        ns |= out[j] || (e[j] & G::FLAG) != 0;
+      }
+      if (wave_ballot(ns)) {
+#pragma unroll
+        for (int j = 0; j < N; ++j)
+          if (out[j] || (e[j] & G::FLAG)) r[j] = search(key[j], e[j], out[j]);
+      }
     }
 #pragma unroll
     for (int j = 0; j < N; ++j) b[j] = range_bucket(r[j], key[j]);

@@ -88,6 +88,22 @@ int run(uint32_t mode, uint32_t S, uint32_t P, uint32_t B, int kind, uint64_t se
     uint32_t got = mp.windowed() ? mp.bucket<true>(key) : mp.bucket<false>(key);
     if (got != want) { if (bad < 5) printf("key %08x got %u want %u (r=%u)\n", key, got, want, r); bad++; }
   }
+  // the four-key form the streaming kernels use (its level-2 and search
+  // blocks are shared by the four keys)
+  for (int t = 0; t < 50000; ++t) {
+    uint32_t k4[4], b4[4], w4[4];
+    for (int j = 0; j < 4; ++j) {
+      const int kind4 = (int)(rng() % 3);
+      k4[j] = kind4 == 0 ? (uint32_t)rng() : kind4 == 1 ? KeyTraits<float>::enc(x[rng() % n])
+                                                        : starts[rng() % R] + (uint32_t)(rng() % 3) - 1;
+      const uint32_t r = std::upper_bound(starts.begin(), starts.end(), k4[j]) - starts.begin() - 1;
+      w4[j] = ranges[r].base + map_off<uint32_t>(k4[j], ranges[r].lo, ranges[r].sh, ranges[r].mult);
+    }
+    threadIdx = dim3(0);
+    if (mp.windowed()) mp.buckets<4, true>(k4, b4); else mp.buckets<4, false>(k4, b4);
+    for (int j = 0; j < 4; ++j)
+      if (b4[j] != w4[j]) { if (bad < 5) printf("4-key form: key %08x got %u want %u\n", k4[j], b4[j], w4[j]); bad++; }
+  }
   printf("mode %u S %u P %u kind %d: R=%u B=%u %s\n", mode, S, P, kind, R, ctl.B, bad ? "FAIL" : "ok");
   return bad ? 1 : 0;
 }
