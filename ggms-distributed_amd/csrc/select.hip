@@ -708,7 +708,9 @@ static ggms_status wait_mirror(volatile uint32_t* seq, uint32_t want, hipStream_
       if (q == hipSuccess) return *seq == want ? GGMS_OK : GGMS_EDEVICE;
       if (q != hipErrorNotReady) return GGMS_EDEVICE;
     }
+#if defined(__x86_64__) || defined(__i386__)
     __builtin_ia32_pause();
+#endif
   }
   std::atomic_thread_fence(std::memory_order_acquire);
   return GGMS_OK;
