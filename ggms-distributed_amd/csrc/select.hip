@@ -488,6 +488,7 @@ static ggms_status launch_fixed(const Config& c, const Layout& L, char* W, uint3
   c0.G = G0;
   c0.nan_lo_bucket = kNone32;
   c0.nan_hi_bucket = kNone32;
+  c0.hot_bucket = kNone32;
   // the status words start clear (a fill kernel: capturable, unlike a memset)
   k_fill_u32<<<1, 64, 0, st>>>(flags, 3, 0u);
   HIP_TRY(hipGetLastError());
@@ -807,6 +808,7 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
   c0.G = G0;
   c0.nan_lo_bucket = kNone32;
   c0.nan_hi_bucket = kNone32;
+  c0.hot_bucket = kNone32;
   // setup (control blocks, ks -> ranks): its own kernel, or folded into the
   // first-pass planning kernel when the ks are few enough for one workgroup
   const uint32_t ctl_words = (uint32_t)((c.max_passes + 2) * ctlb / 4);
@@ -1076,6 +1078,7 @@ static ggms_status run_first_pass(ggms_ctx* ctx, const T* d_vec, uint64_t n64, c
   c0.G = G0;
   c0.nan_lo_bucket = kNone32;
   c0.nan_hi_bucket = kNone32;
+  c0.hot_bucket = kNone32;
   memcpy(&hctl[2], &c0, ctlb);
   HIP_TRY(hipMemcpyAsync(ctl, &hctl[2], ctlb, hipMemcpyHostToDevice, st));
   Plan0Args pa{};

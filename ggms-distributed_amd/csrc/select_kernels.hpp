@@ -73,7 +73,10 @@ struct PassCtl {
   // ranges of the keys outside that window: those below it lie in ranges
   // [0, lut_r_below], those above it in [lut_r_above, R - 1]
   uint32_t lut_r_below, lut_r_above;
-  uint32_t pad[6];
+  // first pass: bucket of a dominant value (a one-key range holding at least
+  // a quarter of the sample), counted in registers by k_count0; else kNone32
+  uint32_t hot_bucket;
+  uint32_t pad[5];
 };
 
 constexpr uint32_t kSetupKsPerThread = 8;
@@ -1243,13 +1246,19 @@ __global__ void __launch_bounds__(1024) k_count0(const T* __restrict__ x, const 
   mp.load(lut, ranges, ctl->R);
   mp.set_window(ctl);
   __syncthreads();
-  auto run = [&](auto win) {
+  // With a dominant value (its bucket hb), a key of it only adds to a
+  // register counter: no LDS atomic and no wave-wide peeling for most keys;
+  // each wave adds its lanes' counts to the histogram once per tile.
+  const uint32_t hb = ctl->hot_bucket;
+  auto run = [&](auto win, auto hot) {
     constexpr bool W = decltype(win)::value;
+    constexpr bool H = decltype(hot)::value;
     for (uint32_t h = 0; h < tiles_per_wg && w0 + h < G; ++h) {
       uint32_t* hist = hist0 + h * B;
       uint32_t t0 = (w0 + h) * tile;
       uint32_t t1 = tile_end(t0, tile, n);
       constexpr int N = Vec16<T>::N;
+      uint32_t hot_cnt = 0;
       stream_tile_vec_any<T, true>(
           x, t0, t1,
           [&](const T (&v)[N]) {
@@ -1258,13 +1267,34 @@ __global__ void __launch_bounds__(1024) k_count0(const T* __restrict__ x, const 
             for (int j = 0; j < N; ++j) k[j] = KeyTraits<T>::enc(v[j]);
             uint32_t b[N];
             mp.template buckets<N, W>(k, b);
-            lds_count_n<N>(hist, b);
+            if constexpr (H) {
+#pragma unroll
+              for (int j = 0; j < N; ++j) {
+                if (b[j] == hb) ++hot_cnt;
+                else atomicAdd(&hist[b[j]], 1u);
+              }
+            } else {
+              lds_count_n<N>(hist, b);
+            }
           },
-          [&](T v) { lds_count(hist, mp.template bucket<W>(KeyTraits<T>::enc(v))); });
+          [&](T v) {
+            const uint32_t b = mp.template bucket<W>(KeyTraits<T>::enc(v));
+            if (H && b == hb) ++hot_cnt;
+            else lds_count(hist, b);
+          });
+      if constexpr (H) {
+        const uint32_t wsum = readlane_u32(wave_incl_scan_u32(hot_cnt), 63);
+        if (lane_id() == 0 && wsum) atomicAdd(&hist[hb], wsum);
+      }
     }
   };
-  if (mp.windowed()) run(std::true_type{});
-  else run(std::false_type{});
+  if (hb != kNone32) {
+    if (mp.windowed()) run(std::true_type{}, std::true_type{});
+    else run(std::false_type{}, std::true_type{});
+  } else {
+    if (mp.windowed()) run(std::true_type{}, std::false_type{});
+    else run(std::false_type{}, std::false_type{});
+  }
   __syncthreads();
   for (uint32_t h = 0; h < tiles_per_wg && w0 + h < G; ++h) {
     uint32_t* row = rows + (size_t)(w0 + h) * B;

@@ -310,7 +310,11 @@ __global__ void __launch_bounds__(1024) k_plan0(const T* __restrict__ x, Plan0Ar
   // longest run of equal pivots (length - 1) << 16 | its first pivot: the
   // most frequent sample value, which the cell-table window puts on a cell edge
   __shared__ uint32_t hot_run;
-  if (threadIdx.x == 0) hot_run = 0;
+  __shared__ uint32_t hot_bucket;  // bucket of a dominant value's one-key range (k_count0 counts it in registers)
+  if (threadIdx.x == 0) {
+    hot_run = 0;
+    hot_bucket = kNone32;
+  }
   __syncthreads();
   {
     enum : uint32_t { kSingle = 0, kEdge = 1, kInterior = 2, kNan = 3 };
@@ -360,6 +364,9 @@ __global__ void __launch_bounds__(1024) k_plan0(const T* __restrict__ x, Plan0Ar
   }
   __syncthreads();
   const uint32_t R = bcast_R;
+  // a value holding at least a quarter of the pivots is dominant
+  const bool hot = hot_run && 4 * ((hot_run >> 16) + 1) >= Q;
+  const K hv = hot ? q[hot_run & 0xFFFFu] : (K)0;
   // 4. map parameters (parallel), bucket bases (block scan), global copy
   uint32_t per = (R + blockDim.x - 1) / blockDim.x;
   uint32_t r0 = threadIdx.x * per, r1 = r0 + per < R ? r0 + per : R;
@@ -384,6 +391,7 @@ __global__ void __launch_bounds__(1024) k_plan0(const T* __restrict__ x, Plan0Ar
     rr.seg_end = 0;
     off += rr.nb;
     ranges[r] = rr;
+    if (hot && rr.lo == hv && rr.hi == hv) hot_bucket = rr.base;
   }
   if (threadIdx.x == 0) {
     ctl->R = R;
@@ -430,6 +438,7 @@ __global__ void __launch_bounds__(1024) k_plan0(const T* __restrict__ x, Plan0Ar
   }
   const uint32_t l2 = build_lut_counted<K>(rlo, R, lut, cells, wbase, wsh);
   if (threadIdx.x == 0) {
+    ctl->hot_bucket = hot_bucket;
     ctl->l2_used = l2;
     ctl->lut_base_lo = (uint32_t)(uint64_t)wbase;
     ctl->lut_base_hi = (uint32_t)((uint64_t)wbase >> 32);

@@ -73,6 +73,23 @@ int run(uint32_t mode, uint32_t S, uint32_t P, uint32_t B, int kind, uint64_t se
       bad++;
     }
   }
+  // the dominant-value bucket: kinds 1 (90 % one value) and 6 (99 %) name the
+  // one-key range of that value; inputs without a dominant value name none
+  {
+    const float dom = kind == 1 ? 2.5f : 1.0f;
+    if (kind == 1 || kind == 6) {
+      const uint32_t kd = KeyTraits<float>::enc(dom);
+      const uint32_t r = std::upper_bound(starts.begin(), starts.end(), kd) - starts.begin() - 1;
+      if (ctl.hot_bucket == kNone32 || ranges[r].lo != kd || ranges[r].hi != kd || ranges[r].base != ctl.hot_bucket) {
+        printf("kind %d: dominant bucket %u, range [%08x, %08x] base %u\n", kind, ctl.hot_bucket, ranges[r].lo,
+               ranges[r].hi, ranges[r].base);
+        bad++;
+      }
+    } else if ((kind == 0 || kind == 4) && ctl.hot_bucket != kNone32) {
+      printf("kind %d: dominant bucket %u without a dominant value\n", kind, ctl.hot_bucket);
+      bad++;
+    }
+  }
   for (uint32_t r = 0; r < R; ++r) mp.store_param(r, ranges[r]);
   for (int t = 0; t < 200000; ++t) {
     uint32_t key = (uint32_t)rng();
