@@ -665,7 +665,13 @@ static ggms_status launch_segmented_first_pass(const Config& c, const Layout& L,
   const size_t lds1 = Pass0Map<K>::param_bytes(c.R0_bound) + S::pass0_bytes(c.B0_bound);
   LDS_CHECK(lds1 + LutGeom<K>::bytes() + 5 * 1024);
   set_lds_attr((const void*)k_group_pass0<T>, lds1);
+  const size_t lds_sp = Pass0Map<K>::param_bytes(c.R0_bound) + ((size_t)c.B0_bound * 2 + 15) / 16 * 16;
+  set_lds_attr((const void*)k_group_scatter<T>, lds_sp);
   prof.begin(GGMS_KERNEL_COMPACT0, 0);
+  // the plan picks one of the two (ctl->grp_sparse); the other returns at once
+  k_group_scatter<T><<<G0, 1024, lds_sp, st>>>(d_vec, rc, lut, cp, (const uint32_t*)P(bRows), pl.slot_of_bucket,
+                                               pl.slot_bucket, (uint32_t*)P(bGrpCursor), scratch,
+                                               (uint8_t*)P(bGrpIds));
   k_group_pass0<T><<<G0, S::kThreads, lds1, st>>>(d_vec, rc, lut, cp, (const uint32_t*)P(bRows), pl.slot_of_bucket,
                                                   pl.slot_bucket, (uint32_t*)P(bGrpCursor), scratch,
                                                   (uint8_t*)P(bGrpIds));
