@@ -169,7 +169,6 @@ struct ggms_ctx {
   void* pinned = nullptr;  // host staging (ctl blocks, ks, outputs)
   void* pinned_dev = nullptr;  // the same memory as the device addresses it
   size_t pinned_bytes = 0;
-  hipEvent_t plan_ev = nullptr;  // kept for callers of older layouts (unused by the pass loop)
   uint32_t mirror_seq = 0;       // last sequence number a plan kernel was asked to publish
   ggms_stats stats{};
   std::vector<hipEvent_t> events;  // timing events (opts.profile)
@@ -184,8 +183,8 @@ static ggms_status ensure_pinned(ggms_ctx* c, size_t bytes) {
   c->pinned = nullptr;
   c->pinned_bytes = 0;
   size_t b = std::max<size_t>(bytes, 1 << 16);
-  // coherent: the plan kernel writes control blocks here for the host to read
-  // as soon as the kernel has finished
+  // coherent: the plan kernel writes control blocks here, then a sequence
+  // word the host polls, so the host reads them while the next kernels run
   if (hipHostMalloc(&c->pinned, b, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess) return GGMS_ENOMEM;
   if (hipHostGetDevicePointer(&c->pinned_dev, c->pinned, 0) != hipSuccess) {
     (void)hipHostFree(c->pinned);
@@ -193,7 +192,6 @@ static ggms_status ensure_pinned(ggms_ctx* c, size_t bytes) {
     return GGMS_EDEVICE;
   }
   c->pinned_bytes = b;
-  if (!c->plan_ev && hipEventCreateWithFlags(&c->plan_ev, hipEventDisableTiming) != hipSuccess) return GGMS_EDEVICE;
   return GGMS_OK;
 }
 
@@ -1319,7 +1317,6 @@ void ggms_ctx_destroy(ggms_ctx* c) {
   (void)hipSetDevice(c->device);
   if (c->ws.base) (void)hipFree(c->ws.base);
   if (c->pinned) (void)hipHostFree(c->pinned);
-  if (c->plan_ev) (void)hipEventDestroy(c->plan_ev);
   for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
   (void)hipSetDevice(cur);
   delete c;
