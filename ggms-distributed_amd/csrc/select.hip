@@ -86,16 +86,16 @@ static Config resolve_config(ggms_alg alg, uint64_t n, uint32_t nk, const ggms_o
   // The iterative forms carry every first-pass k-bucket into one merged second
   // pass, which re-buckets them all under one bucket budget: right for few ks,
   // where the k-buckets are few and large.  With many ks (automatic: at least
-  // one k per five first-pass buckets) the k-buckets are many and most of them
+  // one k per six first-pass buckets) the k-buckets are many and most of them
   // fit the finish, so the first pass writes each into its own segment, those
   // within the finish capacity are finished at once and only the larger ones
   // are re-bucketed, each with its own share of the budget.  Measured at 2^28
-  // fp32 N(0,1), merged / segmented: 2048 ks 0.94 / 1.11 ms, 3072 ks
-  // 1.48 / 1.23, 8192 ks (mixture) 3.40 / 2.33.
+  // fp32 N(0,1), merged / segmented: 1536 ks 0.78 / 1.07 ms, 2048 ks
+  // 1.01 / 1.17, 2560 ks 1.30 / 1.25, 3072 ks 1.50 / 1.22.
   const uint32_t B_auto = o.num_buckets ? o.num_buckets : 14336u;
   if (c.one_pass || o.first_pass == GGMS_FIRST_PASS_SEGMENTED) c.seg_first = true;
   else if (o.first_pass == GGMS_FIRST_PASS_MERGED) c.seg_first = false;
-  else c.seg_first = (uint64_t)nk * 5 >= B_auto;
+  else c.seg_first = (uint64_t)nk * 6 >= B_auto;
   if (o.splitter == GGMS_SPLIT_LEARNED) c.mode = kModeLearned;
   if (o.splitter == GGMS_SPLIT_SKETCH) c.mode = kModeSketch;
   // 0 = automatic: 14336 for every splitter (the reference uses 8192).  More
