@@ -86,16 +86,17 @@ static Config resolve_config(ggms_alg alg, uint64_t n, uint32_t nk, const ggms_o
   // The iterative forms carry every first-pass k-bucket into one merged second
   // pass, which re-buckets them all under one bucket budget: right for few ks,
   // where the k-buckets are few and large.  With many ks (automatic: at least
-  // one k per six first-pass buckets) the k-buckets are many and most of them
+  // one k per four first-pass buckets) the k-buckets are many and most of them
   // fit the finish, so the first pass writes each into its own segment, those
   // within the finish capacity are finished at once and only the larger ones
   // are re-bucketed, each with its own share of the budget.  Measured at 2^28
-  // fp32 N(0,1), merged / segmented: 1536 ks 0.78 / 1.07 ms, 2048 ks
-  // 1.01 / 1.17, 2560 ks 1.30 / 1.25, 3072 ks 1.50 / 1.22.
+  // fp32 N(0,1), merged / segmented: 2048 ks 0.89 / 1.17 ms, 3072 ks
+  // 1.17 / 1.23, 3584 ks 1.28 / 1.27, 4096 ks 1.37 / 1.32, 6144 ks 2.44 / 1.49
+  // (above ~5000 ks the pass-1 range parameters no longer fit LDS).
   const uint32_t B_auto = o.num_buckets ? o.num_buckets : 14336u;
   if (c.one_pass || o.first_pass == GGMS_FIRST_PASS_SEGMENTED) c.seg_first = true;
   else if (o.first_pass == GGMS_FIRST_PASS_MERGED) c.seg_first = false;
-  else c.seg_first = (uint64_t)nk * 6 >= B_auto;
+  else c.seg_first = (uint64_t)nk * 4 >= B_auto;
   if (o.splitter == GGMS_SPLIT_LEARNED) c.mode = kModeLearned;
   if (o.splitter == GGMS_SPLIT_SKETCH) c.mode = kModeSketch;
   // 0 = automatic: 14336 for every splitter (the reference uses 8192).  More
@@ -441,14 +442,23 @@ static PlanArgs<K> plan_args_common(const Config& c, const Layout& L, char* W, u
   return pl;
 }
 
+// The pass-1 kernels hold the pass-1 range parameters in LDS whenever their
+// tables with them fit this (they run one workgroup per CU either way);
+// beyond it every key reads its range's parameters from the caches.  At 2^28
+// N(0,1) with 2048 ks (1900 pass-1 ranges) the parameters took the pass-1
+// kernels past the former 112 KB limit: compaction 0.27 ms with them in
+// global memory.
+constexpr size_t kPass1ParamsLds = kLdsLimit;
+
 // Dynamic LDS the pass-1 kernels are given in a schedule fixed before the
 // first pass has run (bounds instead of the first pass's exact table sizes):
-// their tables without the pass-1 range parameters, and at least the 112 KB
-// up to which the parameters join them (the kernels decide that on the device).
+// their tables without the pass-1 range parameters, and at least the
+// kPass1ParamsLds up to which the parameters join them (the kernels decide
+// that on the device).
 template <class K>
 static size_t pass1_lds_fixed(const Config& c, size_t own) {
   const size_t base = own + pass1_map_bytes<K>(c.B0_bound, c.R0_bound, LutGeom<K>::L2TABLES, 0);
-  return std::max<size_t>(base, 112 * 1024);
+  return std::max<size_t>(base, kPass1ParamsLds);
 }
 
 // The whole call enqueued on `st` with no host reads between kernels (a
@@ -563,7 +573,7 @@ static ggms_status launch_fixed(const Config& c, const Layout& L, char* W, uint3
       set_lds_attr((const void*)k_count1<K>, lds);
       prof.begin(GGMS_KERNEL_COUNTS, (int)pass);
       k_count1<K><<<Gn, NT, lds, st>>>(src, ctl, (K*)P(bLut), rng_first, (uint64_t*)P(bRank0),
-                                        (MapParam<K>*)P(bParams1), 112u * 1024u, cp, rows);
+                                        (MapParam<K>*)P(bParams1), (uint32_t)kPass1ParamsLds, cp, rows);
       prof.end();
     } else {
       const size_t lds = (size_t)B_bound * 4;
@@ -616,7 +626,7 @@ static ggms_status launch_fixed(const Config& c, const Layout& L, char* W, uint3
       set_lds_attr((const void*)k_compact1<K>, lds);
       prof.begin(GGMS_KERNEL_COMPACTS, (int)pass);
       k_compact1<K><<<Gn, NT, lds, st>>>(src, ctl, (K*)P(bLut), rng_first, (uint64_t*)P(bRank0),
-                                          (MapParam<K>*)P(bParams1), 112u * 1024u, cp, rows, pl.slot_of_bucket,
+                                          (MapParam<K>*)P(bParams1), (uint32_t)kPass1ParamsLds, cp, rows, pl.slot_of_bucket,
                                           pl.slot_dst, pl.slot_bucket, dst);
       prof.end();
     } else {
@@ -881,12 +891,12 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
       uint32_t* wgf_next = (uint32_t*)P(bWgFirst) + (size_t)((pass + 1) & 1) * c.G_max;
       const uint32_t NT = c.NT;
       // pass-1 range parameters live in LDS unless that would push either
-      // pass-1 kernel past 112 KB (then they are read from global memory)
+      // pass-1 kernel past kPass1ParamsLds (then they are read from global memory)
       uint32_t p1_lds = 0;
       if (pass == 1 && !c.seg_first) {
         size_t cnt = ((size_t)B_bound * 4 + 15) / 16 * 16 + pass1_map_bytes<K>(B0, R0, L2, R_bound);
         size_t cmp = compact_lds_bytes(B_bound, U_slot_bound + 4) + pass1_map_bytes<K>(B0, R0, L2, R_bound);
-        p1_lds = std::max(cnt, cmp) <= 112 * 1024 ? 1u : 0u;
+        p1_lds = std::max(cnt, cmp) <= kPass1ParamsLds ? 1u : 0u;
       }
       // count
       if (pass == 0) {

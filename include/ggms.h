@@ -82,7 +82,7 @@ typedef struct {
  *   SEGMENTED: every k-bucket is compacted into a segment of its own; those
  *              within the finish capacity are finished right away, the others
  *              re-bucketed segment by segment; many ks.
- *   AUTO:      SEGMENTED when there is at least one k per six first-pass
+ *   AUTO:      SEGMENTED when there is at least one k per four first-pass
  *              buckets, MERGED otherwise.  Results are identical either way. */
 typedef enum { GGMS_FIRST_PASS_AUTO = 0, GGMS_FIRST_PASS_MERGED = 1, GGMS_FIRST_PASS_SEGMENTED = 2 } ggms_first_pass;
 
