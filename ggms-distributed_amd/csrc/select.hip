@@ -1012,10 +1012,24 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
         // every segment within a quarter of the capacity: half-size workgroups
         // (512 threads, half the LDS) so twice as many run at once
         const bool half = h.small_max <= c.fin_cap / 4;
+        // many small segments beside a few large ones: the large ones first
+        // (full workgroups), then the small ones in half-size workgroups
+        const uint32_t U_grp = h.U_small - h.U_wave - h.U_wide;
+        const bool split = !half && c.fin_cap / 4 >= kFinishSplitLen &&
+                           U_grp - std::min(h.U_fin_big, U_grp) >= kFinishSplitMinSegs;
         prof.begin(GGMS_KERNEL_FINISH, (int)pass);
-        k_finish<T><<<fin_grid, half ? 512 : 1024, half ? fin_lds / 2 : fin_lds, st>>>(
-            dst, cp, pl.small_start, pl.small_len, pl.small_klo, pl.small_khi, pl.small_kstart, pl.small_order, klist,
-            k_rank, k_state, d_out, half ? c.fin_cap / 2 : c.fin_cap, dflag);
+        if (split) {
+          k_finish<T><<<fin_grid, 1024, fin_lds, st>>>(dst, cp, pl.small_start, pl.small_len, pl.small_klo,
+                                                       pl.small_khi, pl.small_kstart, pl.small_order, klist, k_rank,
+                                                       k_state, d_out, c.fin_cap, dflag, 1u);
+          k_finish<T><<<fin_grid, 512, fin_lds / 2, st>>>(dst, cp, pl.small_start, pl.small_len, pl.small_klo,
+                                                          pl.small_khi, pl.small_kstart, pl.small_order, klist, k_rank,
+                                                          k_state, d_out, c.fin_cap / 2, dflag, 2u);
+        } else {
+          k_finish<T><<<fin_grid, half ? 512 : 1024, half ? fin_lds / 2 : fin_lds, st>>>(
+              dst, cp, pl.small_start, pl.small_len, pl.small_klo, pl.small_khi, pl.small_kstart, pl.small_order,
+              klist, k_rank, k_state, d_out, half ? c.fin_cap / 2 : c.fin_cap, dflag);
+        }
         if (h.U_wide > 0)
           k_finish_wide<T><<<std::min<uint32_t>(h.U_wide, (uint32_t)ctx->cus), 1024, 0, st>>>(
               dst, src, cp, pl.small_start, pl.small_len, pl.small_klo, pl.small_khi, pl.small_kstart, pl.small_order,

@@ -79,7 +79,10 @@ struct PassCtl {
   // segmented first pass: nonzero when fewer than a quarter of the keys
   // survive, so the unstaged grouped compaction runs instead of the staged one
   uint32_t grp_sparse;
-  uint32_t pad[4];
+  // finish order: the whole-workgroup segments of at least kFinishSplitLen
+  // keys come first, in [0, U_fin_big)
+  uint32_t U_fin_big;
+  uint32_t pad[3];
 };
 
 constexpr uint32_t kSetupKsPerThread = 8;
@@ -504,6 +507,12 @@ constexpr uint32_t kOrderBins = 35;
 constexpr uint32_t kFinishWaveLen = 4096;
 constexpr uint32_t kFinishWaveKs = 4;
 constexpr uint32_t kFinishWaveMinSegs = 2048;
+// With at least kFinishSplitMinSegs whole-workgroup segments below
+// kFinishSplitLen keys, the host-driven schedule finishes those with 512-thread
+// workgroups and half the LDS (three per CU instead of two) in a launch of
+// their own, after the larger ones.
+constexpr uint32_t kFinishSplitLen = 8192;
+constexpr uint32_t kFinishSplitMinSegs = 2048;
 template <class K>
 __host__ __device__ __forceinline__ bool finish_wide_segment(uint32_t len, K klo, K khi, uint32_t fin_cap) {
   return len > fin_cap || !finish_fits_half<K>(len, klo, khi, fin_cap);
@@ -1773,6 +1782,7 @@ __global__ void __launch_bounds__(1024) k_plan(PlanArgs<typename KeyTraits<T>::K
   }
   __threadfence_block();
   __syncthreads();
+  __shared__ uint32_t s_fin_big;
   if (t == 0) {  // largest size class first
     sh_tot[5] = ord_cnt[0];
     sh_tot[6] = ord_cnt[1];
@@ -1782,6 +1792,9 @@ __global__ void __launch_bounds__(1024) k_plan(PlanArgs<typename KeyTraits<T>::K
       ord_cnt[c] = acc;
       acc += v;
     }
+    // the classes from kFinishSplitLen keys up come first: they end where
+    // the class below them starts
+    s_fin_big = ord_cnt[2 + (kFinishSplitLen >> 10) - 1];
   }
   uint32_t live = 0;
   for (uint32_t k0 = t; k0 < a.nk; k0 += 4 * nt) {  // four ks per thread at a time
@@ -1850,6 +1863,7 @@ __global__ void __launch_bounds__(1024) k_plan(PlanArgs<typename KeyTraits<T>::K
     cc->live_ks = live_tot;
     cc->U_wide = sh_tot[5];
     cc->U_wave = sh_tot[6];
+    cc->U_fin_big = s_fin_big;
     cc->small_max = sh_tot[7];
     uint32_t nan = 0;
     if (ctl.nan_lo_bucket != kNone32) nan += a.totals[ctl.nan_lo_bucket];
@@ -2977,19 +2991,24 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8)
                                                  const uint32_t* __restrict__ small_order,
                                                  const uint32_t* __restrict__ klist, uint32_t* __restrict__ k_rank,
                                                  uint32_t* __restrict__ k_state, T* __restrict__ out,
-                                                 uint32_t fin_cap, uint32_t* __restrict__ err) {
+                                                 uint32_t fin_cap, uint32_t* __restrict__ err,
+                                                 uint32_t part = 0) {
   using K = typename KeyTraits<T>::K;
   using H = typename HalfKey<K>::T;
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ SelectLds L;
-  // finish order: [whole-workgroup segments | U_wave one-wave segments | U_wide (k_finish_wide)]
+  // finish order: [whole-workgroup segments | U_wave one-wave segments | U_wide (k_finish_wide)];
+  // part 0 takes all of it, part 1 only the whole-workgroup segments of at
+  // least kFinishSplitLen keys, part 2 the rest (see kFinishSplitMinSegs)
   const uint32_t U_small = ctl->U_small, U_wave = ctl->U_wave;
   const uint32_t U_grp = U_small - U_wave - ctl->U_wide;
+  const uint32_t split = ctl->U_fin_big < U_grp ? ctl->U_fin_big : U_grp;
+  const uint32_t jj_lo = part == 2 ? split : 0u, jj_hi = part == 1 ? split : U_grp;
   // the first digit's histogram starts cleared; each segment that counts
   // into it clears it again once done
   select_zero_first(L);
   __syncthreads();
-  for (uint32_t jj = blockIdx.x; jj < U_grp; jj += gridDim.x) {
+  for (uint32_t jj = jj_lo + blockIdx.x; jj < jj_hi; jj += gridDim.x) {
     const uint32_t j = small_order ? small_order[jj] : jj;
     const uint32_t st = small_start[j], len = small_len[j];
     if (len > fin_cap) {  // cannot happen: such segments are ordered into k_finish_wide's range; reported
@@ -3029,7 +3048,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8)
   // segments of at most kFinishWaveLen keys and kFinishWaveKs ks: one wave
   // each, radix-selected from global memory (L2) with a 256-bin histogram of
   // the wave's own in LDS, no workgroup barrier
-  if (U_wave == 0) return;
+  if (U_wave == 0 || part == 1) return;
   const uint32_t w = threadIdx.x >> 6, nw = blockDim.x >> 6;
   uint32_t* wh = reinterpret_cast<uint32_t*>(smem) + w * 256;
   uint32_t* bc = reinterpret_cast<uint32_t*>(smem) + nw * 256 + w * 2;
