@@ -378,3 +378,32 @@ def test_first_pass_layouts(gpu_ctx, dt, dist, p0):
                 bad = np.nonzero(_bits(got) != _bits(want))[0]
                 assert bad.size == 0, f"{kdist}/{nk} {alg} first_pass={fp} {extra}: {bad.size} wrong, first k={ks[bad[0]]}"
         print(f"{dt}/{dist}/{kdist}: exact", flush=True)
+
+
+@pytest.mark.parametrize("dt", ["float32", "float64", "int32"])
+def test_dominant_value_counted_in_registers(gpu_ctx, dt):
+    """A value holding ~40 % of the keys beside values of both signs (so the
+    first-pass cell table covers the whole key space, no window): the planning
+    kernel reports its one-key range's bucket and the first-pass count tallies
+    that bucket in registers.  Both first-pass layouts, few and many ks, every
+    answer against sort-and-choose (including ranks inside the dominant run)."""
+    import ggms
+    import torch
+    rng = np.random.default_rng(77)
+    n = (1 << 22) + 3
+    if dt == "int32":
+        v = rng.integers(-(1 << 30), 1 << 30, n, dtype=np.int64).astype(np.int32)
+        v[rng.random(n) < 0.4] = -12345
+    else:
+        v = rng.standard_normal(n).astype(NP[dt])
+        v[rng.random(n) < 0.4] = NP[dt](0.75)
+    x = torch.from_numpy(v).cuda()
+    for kdist, nk in (("uniform_random", 1001), ("even", 8192)):
+        ks = ggms.generate_ks(kdist, nk, n, 5 + nk)
+        want = oracle.sort_and_choose(v, ks, threads=16)
+        for fp in (1, 2):
+            for sp in ("linear", "sketch"):
+                got = gpu_ctx.select(x, ks, "iterative", ggms.default_opts(first_pass=fp, splitter=sp))
+                bad = np.nonzero(_bits(got) != _bits(want))[0]
+                assert bad.size == 0, f"{dt} {kdist}/{nk} first_pass={fp} {sp}: {bad.size} wrong, first k={ks[bad[0]]}"
+    print(f"{dt}: exact", flush=True)
