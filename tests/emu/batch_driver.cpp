@@ -67,7 +67,9 @@ static int run_batch(const std::vector<Problem>& ps, uint32_t wgs, uint32_t thre
   std::vector<uint32_t> order(P);
   for (uint32_t p = 0; p < P; ++p) order[p] = p;
   std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return off[a + 1] - off[a] > off[b + 1] - off[b]; });
-  uint32_t grid = std::min(wgs, P);
+  // workgroups beyond the problems find the queue empty at once and take the
+  // kernel's exit protocol beside the busy ones, as most of a GPU grid does
+  uint32_t grid = std::min(std::min(wgs, P + 3), EMU_MAX_WG);
   uint32_t queue[64] = {0};
   std::vector<uint32_t> status(P, 99);
   std::vector<uint32_t> sA(off[P] + 4), sB(off[P] + 4);
@@ -167,5 +169,6 @@ int main(int argc, char** argv) {
     bad += run_batch(ps, 256, 256, lab);
     g_override = ggms_batch_class{};
   }
+  emu_print_wave_stats();
   return bad ? 1 : 0;
 }

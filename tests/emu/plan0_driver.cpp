@@ -3,20 +3,6 @@
 #include <random>
 #include <cmath>
 #include "splitter.hpp"
-// shfl_xor among 32-lane groups: emulate via a per-wave exchange (all lanes call)
-struct XSync { std::mutex m; std::condition_variable cv; uint32_t arrived = 0, gen = 0; float v[64]; };
-static XSync g_x[16];
-float __shfl_xor(float v, int mask, int width) {
-  uint32_t w = threadIdx.x >> 6, l = threadIdx.x & 63; XSync& s = g_x[w];
-  std::unique_lock<std::mutex> lk(s.m);
-  uint32_t g = s.gen; s.v[l] = v;
-  if (++s.arrived == 64) { s.arrived = 0; s.gen++; s.cv.notify_all(); } else s.cv.wait(lk, [&] { return s.gen != g; });
-  float r = s.v[(l & ~(uint32_t)(width - 1)) | ((l ^ mask) & (width - 1))];
-  // second phase so nobody overwrites v before all have read
-  g = s.gen;
-  if (++s.arrived == 64) { s.arrived = 0; s.gen++; s.cv.notify_all(); } else s.cv.wait(lk, [&] { return s.gen != g; });
-  return r;
-}
 using namespace ggms;
 int run(uint32_t mode, uint32_t S, uint32_t P, uint32_t B, int kind, uint64_t seed) {
   std::mt19937_64 rng(seed);
@@ -91,6 +77,7 @@ int run(uint32_t mode, uint32_t S, uint32_t P, uint32_t B, int kind, uint64_t se
     }
   }
   for (uint32_t r = 0; r < R; ++r) mp.store_param(r, ranges[r]);
+  emu_solo = true;  // the map checked from this thread: a wave of one lane
   for (int t = 0; t < 200000; ++t) {
     uint32_t key = (uint32_t)rng();
     if (t % 2) key = KeyTraits<float>::enc(x[rng() % n]);
@@ -121,6 +108,7 @@ int run(uint32_t mode, uint32_t S, uint32_t P, uint32_t B, int kind, uint64_t se
     for (int j = 0; j < 4; ++j)
       if (b4[j] != w4[j]) { if (bad < 5) printf("4-key form: key %08x got %u want %u\n", k4[j], b4[j], w4[j]); bad++; }
   }
+  emu_solo = false;
   printf("mode %u S %u P %u kind %d: R=%u B=%u %s\n", mode, S, P, kind, R, ctl.B, bad ? "FAIL" : "ok");
   return bad ? 1 : 0;
 }

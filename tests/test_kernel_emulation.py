@@ -1,6 +1,7 @@
 """CPU emulation of device kernels (no GPU): the kernel sources are compiled
 with g++ against a small HIP shim (tests/emu/hip/hip_runtime.h) and run with
-one thread per work-item, checked barriers and one-lane waves.  This covers
+one thread per work-item, checked barriers and 64-lane waves (ballots,
+readlanes, shuffles and scans are rendezvous of a wave's active lanes).  This covers
 kernel logic that the GPU parity tests cannot isolate: barrier agreement on
 every path (including the error paths), the batched kernel's planning and
 work stack, the LDS sort network, the finish's multi-rank radix select and the
@@ -114,3 +115,7 @@ def test_batched_kernel_paths(gen_dir):
     re-bucketing work stack."""
     out = _build_and_run(gen_dir, "batch_driver.cpp", [256])
     assert out.count(", ok") == 5
+    # the wave operations ran with many lanes at once, and with some lanes
+    # masked off (divergent branches, loops left early), as on the GPU
+    m = re.search(r"waves: (\d+) operations, (\d+) with several active lanes, (\d+) with some lanes inactive", out)
+    assert m and int(m.group(2)) > 1000 and int(m.group(3)) > 100, out[-500:]
