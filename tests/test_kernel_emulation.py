@@ -80,13 +80,15 @@ def gen_dir(tmp_path_factory):
     return str(d)
 
 
-def _build_and_run(gen_dir, driver, args=(), timeout=900):
-    exe = os.path.join(gen_dir, driver.replace(".cpp", ""))
-    cmd = ["g++", "-std=c++20", "-O2", "-fno-strict-aliasing", "-pthread", "-I", EMU, "-I", os.path.join(ROOT, "include"), "-I", gen_dir,
+def _build_and_run(gen_dir, driver, args=(), timeout=900, asan=False):
+    exe = os.path.join(gen_dir, driver.replace(".cpp", "") + ("_asan" if asan else ""))
+    opt = ["-O1", "-g", "-fsanitize=address", "-fno-omit-frame-pointer"] if asan else ["-O2"]
+    cmd = ["g++", "-std=c++20", *opt, "-fno-strict-aliasing", "-pthread", "-I", EMU, "-I", os.path.join(ROOT, "include"), "-I", gen_dir,
            os.path.join(EMU, driver), "-o", exe]
     r = subprocess.run(cmd, capture_output=True, text=True)
     assert r.returncode == 0, r.stderr[-4000:]
-    r = subprocess.run([exe, *map(str, args)], capture_output=True, text=True, timeout=timeout)
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0") if asan else None
+    r = subprocess.run([exe, *map(str, args)], capture_output=True, text=True, timeout=timeout, env=env)
     print(r.stdout)
     assert r.returncode == 0, r.stdout[-4000:] + r.stderr[-4000:]
     return r.stdout
@@ -119,3 +121,11 @@ def test_batched_kernel_paths(gen_dir):
     # masked off (divergent branches, loops left early), as on the GPU
     m = re.search(r"waves: (\d+) operations, (\d+) with several active lanes, (\d+) with some lanes inactive", out)
     assert m and int(m.group(2)) > 1000 and int(m.group(3)) > 100, out[-500:]
+
+
+def test_batched_kernel_paths_under_address_sanitizer(gen_dir):
+    """The same batches with the driver built with AddressSanitizer (host code): every
+    access of the kernel to its work stacks, scratch regions, outputs and dynamic LDS
+    (allocated at exactly the launch's size) stays inside its buffer."""
+    out = _build_and_run(gen_dir, "batch_driver.cpp", [256], asan=True)
+    assert out.count(", ok") == 5
