@@ -54,7 +54,8 @@ static ggms_batch_class default_class(uint32_t c) {
 struct Problem { std::vector<float> v; std::vector<uint32_t> ks; };
 
 static ggms_batch_class g_override{};
-static int run_batch(const std::vector<Problem>& ps, uint32_t wgs, uint32_t threads, const char* label) {
+static int run_batch(const std::vector<Problem>& ps, uint32_t wgs, uint32_t threads, const char* label,
+                     uint32_t extra = 3) {
   uint32_t P = ps.size();
   std::vector<uint64_t> off(P + 1, 0);
   std::vector<uint32_t> koff(P + 1, 0);
@@ -71,7 +72,7 @@ static int run_batch(const std::vector<Problem>& ps, uint32_t wgs, uint32_t thre
   std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return off[a + 1] - off[a] > off[b + 1] - off[b]; });
   // workgroups beyond the problems find the queue empty at once and take the
   // kernel's exit protocol beside the busy ones, as most of a GPU grid does
-  uint32_t grid = std::min(std::min(wgs, P + 3), EMU_MAX_WG);
+  uint32_t grid = std::min(std::min(wgs, P + extra), EMU_MAX_WG);
   uint32_t queue[64] = {0};
   std::vector<uint32_t> status(P, 99);
   std::vector<uint32_t> sA(off[P] + 4), sB(off[P] + 4);
@@ -117,7 +118,42 @@ static int run_batch(const std::vector<Problem>& ps, uint32_t wgs, uint32_t thre
   return bad;
 }
 
+// A batch written by the test (u32 problems, then per problem u32 n, u32 nk,
+// n floats, nk ks): the GPU tests' own batches, value for value.
+static std::vector<Problem> read_batch(const char* path) {
+  FILE* f = fopen(path, "rb");
+  if (!f) { fprintf(stderr, "cannot open %s\n", path); std::abort(); }
+  auto rd = [&](void* p, size_t n) { if (fread(p, 1, n, f) != n) { fprintf(stderr, "short batch file\n"); std::abort(); } };
+  uint32_t P = 0;
+  rd(&P, 4);
+  std::vector<Problem> ps(P);
+  for (auto& pr : ps) {
+    uint32_t n = 0, nk = 0;
+    rd(&n, 4);
+    rd(&nk, 4);
+    pr.v.resize(n);
+    pr.ks.resize(nk);
+    rd(pr.v.data(), (size_t)n * 4);
+    rd(pr.ks.data(), (size_t)nk * 4);
+  }
+  fclose(f);
+  return ps;
+}
+
 int main(int argc, char** argv) {
+  if (argc > 2) {  // replay batch files: 1024 threads, the library's grid (one workgroup per problem), then three more
+    int bad = 0;
+    for (int i = 2; i < argc; ++i) {
+      const std::vector<Problem> ps = read_batch(argv[i]);
+      char lab[64];
+      snprintf(lab, sizeof lab, "file%d", i - 2);
+      bad += run_batch(ps, 256, 1024, lab, 0);
+      snprintf(lab, sizeof lab, "file%d+3", i - 2);
+      bad += run_batch(ps, 256, 1024, lab, 3);
+    }
+    emu_print_wave_stats();
+    return bad ? 1 : 0;
+  }
   std::mt19937_64 rng(10);
   std::normal_distribution<float> nd;
   int bad = 0;

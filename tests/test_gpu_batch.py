@@ -146,21 +146,31 @@ def test_batch_graph_replays_with_new_values(gpu_ctx):
 
 
 @pytest.mark.skip(reason="a launch over a batch mixing NaN-holding problems with valid ones (and, then, bad ks) did "
-                         "not complete on the GPU once; no cause is found (the kernel emulation runs the same batch "
-                         "with every workgroup concurrent and poisoned LDS and answers it), so the case stays off the "
-                         "GPU")
-def test_batch_nan_problems_are_per_problem(gpu_ctx):
-    """NaN-holding problems (one above and one below the LDS finish size) beside valid
-    ones: each NaN problem reports GGMS_ENAN, every valid one is answered exactly."""
-    import ggms
+                         "not complete on the GPU once; no cause is found (the kernel emulation replays both batches "
+                         "value for value with 64-lane waves, every workgroup concurrent, poisoned LDS and "
+                         "AddressSanitizer, and answers them), so the case stays off the GPU")
+def nan_batch_parts(bad_ks=False):
+    """The NaN batch: NaN-holding problems (one above and one below the LDS finish size)
+    beside valid ones; with bad_ks, the earlier form with two problems of invalid ks
+    (k = 0 and k = n + 1) between them.  (values, ks) per problem, float32."""
     rng = np.random.default_rng(10)
     good = rng.standard_normal(50_000).astype(np.float32)
     nan_big = good.copy()
     nan_big[1234] = np.nan
     nan_small = good[:500].copy()
     nan_small[7] = np.nan
-    parts = [(good, [1, 2, 50_000]), (nan_big, [5]), (good[:1000], [3]), (nan_small, [3]),
-             (good[:20_000], [17, 20_000])]
+    if bad_ks:
+        return [(good, [1, 2, 50_000]), (nan_big, [5]), (good[:1000], [0]), (good[:1000], [1001]),
+                (nan_small, [3]), (good[:20_000], [17, 20_000])]
+    return [(good, [1, 2, 50_000]), (nan_big, [5]), (good[:1000], [3]), (nan_small, [3]),
+            (good[:20_000], [17, 20_000])]
+
+
+def test_batch_nan_problems_are_per_problem(gpu_ctx):
+    """NaN-holding problems (one above and one below the LDS finish size) beside valid
+    ones: each NaN problem reports GGMS_ENAN, every valid one is answered exactly."""
+    import ggms
+    parts = nan_batch_parts()
     x, off, ks, koff = _concat(parts, np.float32)
     pl = _plan(x, off, ks, koff)
     pl.launch()

@@ -123,6 +123,33 @@ def test_batched_kernel_paths(gen_dir):
     assert m and int(m.group(2)) > 1000 and int(m.group(3)) > 100, out[-500:]
 
 
+def _write_batch(path, parts):
+    import numpy as np
+    with open(path, "wb") as f:
+        f.write(np.uint32(len(parts)).tobytes())
+        for v, ks in parts:
+            v = np.ascontiguousarray(v, dtype=np.float32)
+            ks = np.asarray(ks, dtype=np.uint32)
+            f.write(np.array([v.size, ks.size], dtype=np.uint32).tobytes())
+            f.write(v.tobytes())
+            f.write(ks.tobytes())
+
+
+def test_batched_kernel_replays_the_gpu_nan_batches(gen_dir):
+    """The GPU NaN batch value for value (numpy's generator, the library's defaults:
+    1024 threads, one workgroup per problem, seed and class table), and its earlier form
+    with invalid ks passed to the kernel itself: every status and answer right, with
+    AddressSanitizer watching every access."""
+    from test_gpu_batch import nan_batch_parts
+    files = []
+    for bad in (False, True):
+        path = os.path.join(gen_dir, f"nan_batch_{int(bad)}.bin")
+        _write_batch(path, nan_batch_parts(bad_ks=bad))
+        files.append(path)
+    out = _build_and_run(gen_dir, "batch_driver.cpp", [256, *files], asan=True)
+    assert out.count(", ok") == 4, out
+
+
 def test_batched_kernel_paths_under_address_sanitizer(gen_dir):
     """The same batches with the driver built with AddressSanitizer (host code): every
     access of the kernel to its work stacks, scratch regions, outputs and dynamic LDS
