@@ -1491,7 +1491,8 @@ __global__ void __launch_bounds__(1024) k_plan(PlanArgs<typename KeyTraits<T>::K
   __syncthreads();
   block_incl_scan_lds(P, B, scratch);
 
-  // P1: findKBuckets for every live k
+  // P1: findKBuckets for every live k (its rank inside the bucket from the
+  // prefix below the bucket: LDS only)
   for (uint32_t k = t; k < a.nk; k += nt) {
     if (a.k_state[k] != kLive) continue;
     uint32_t g = a.k_rank[k];
@@ -1502,7 +1503,7 @@ __global__ void __launch_bounds__(1024) k_plan(PlanArgs<typename KeyTraits<T>::K
     }
     aidx[lo] = 1;
     a.k_aux[k] = lo;
-    a.k_rank[k] = g - (P[lo] - a.totals[lo]);  // rank inside the bucket (1-based)
+    a.k_rank[k] = g - (lo ? P[lo - 1] : 0u);  // rank inside the bucket (1-based)
   }
   __syncthreads();
   // P is free from here on: reuse it for the ranges' first buckets
