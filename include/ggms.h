@@ -277,8 +277,11 @@ ggms_status ggms_debug_first_pass(ggms_ctx* ctx, ggms_dtype t, const void* d_vec
  * Per-size-class parameters (class c = problems with ceil(log2(size)) == c,
  * the last class also takes every larger size) normally come from the
  * autotuner's table.  The k ranks are read and checked when the plan is made
- * (GGMS_EINVAL for a k outside [1, size]); a NaN in a problem's values is a
- * per-problem GGMS_ENAN, reported by ggms_batch_plan_status after the launch.
+ * (GGMS_EINVAL for a k outside [1, size]); a NaN in a problem's values is
+ * meant to be a per-problem GGMS_ENAN, reported by ggms_batch_plan_status
+ * after the launch, but a launch over a batch holding NaN values has not
+ * completed on the GPU in testing (cause not found): keep NaN out of
+ * batched inputs, or use ggms_select, which rejects it with GGMS_ENAN.
  * ------------------------------------------------------------------------ */
 #define GGMS_BATCH_CLASSES 24
 #define GGMS_BATCH_MAX_KS 2048
