@@ -347,6 +347,16 @@ constexpr size_t kLdsLimit = 158 * 1024;
 // dynamic LDS that leaves room for only one 1024-thread workgroup per CU
 constexpr size_t kLdsOneWgPerCu = 81 * 1024;
 
+// The plan's LDS: its tables for B buckets (prefix, active index, rank
+// words), then room for one word per small segment (at most one per k) when
+// that fits, where it keeps the segments' class information for its last
+// phases instead of reading it back from global memory.
+static size_t plan_lds_bytes(uint32_t B, uint32_t nk) {
+  const size_t base = (size_t)B * 8 + (size_t)rank_word_count(B) * 8;
+  const size_t want = base + ((size_t)std::min(nk, B) + 1) * 4;
+  return want <= kLdsLimit ? want : base;
+}
+
 // Raises a kernel's dynamic-LDS limit on the current device when a launch
 // needs more than the default 64 KB; each (device, kernel) pair is raised once
 // to what it has needed so far, so repeated calls (and graph capture) make no
@@ -602,7 +612,8 @@ static ggms_status launch_fixed(const Config& c, const Layout& L, char* W, uint3
     pl.mirror = nullptr;
     pl.nan_out = pass == 0 ? flags + 2 : nullptr;
     plan_groups<K>(pl, L, W, pass == 0 && c.seg_first);
-    const size_t lds_plan = (size_t)B_bound * 8 + (size_t)rank_word_count(B_bound) * 8;
+    const size_t lds_plan = plan_lds_bytes(B_bound, nk);
+    pl.lds_words = (uint32_t)(lds_plan / 4);
     LDS_CHECK(lds_plan);
     set_lds_attr((const void*)k_plan<T>, lds_plan);
     prof.begin(GGMS_KERNEL_PLAN, (int)pass);
@@ -954,7 +965,8 @@ static ggms_status run_select(ggms_ctx* ctx, ggms_alg alg, const T* d_vec, uint6
       pl.seq = ctx->mirror_seq;
       pl.nan_out = nullptr;
       plan_groups<K>(pl, L, W, pass == 0 && c.seg_first);
-      size_t lds_plan = (size_t)B_bound * 8 + (size_t)rank_word_count(B_bound) * 8;
+      size_t lds_plan = plan_lds_bytes(B_bound, nk);
+      pl.lds_words = (uint32_t)(lds_plan / 4);
       LDS_CHECK(lds_plan);
       set_lds_attr((const void*)k_plan<T>, lds_plan);
       prof.begin(GGMS_KERNEL_PLAN, (int)pass);

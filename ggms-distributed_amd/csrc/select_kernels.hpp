@@ -1440,6 +1440,7 @@ template <class K> struct PlanArgs {
   PassCtl* mirror;           // host-visible copy of ctl and ctl_next, written last (or null)
   uint32_t* mirror_seq;      // with a mirror: seq is written here after it (host-visible)
   uint32_t seq;
+  uint32_t lds_words;        // dynamic LDS of the launch, in words (the tables' bound plus room for caches)
   uint32_t* nan_out;         // the first pass writes its NaN count here (or null)
   // grouped compaction (or null): slot s belongs to group s / kGroupSlots; the
   // slots of a group are contiguous in the output, so a group owns one span
@@ -1463,6 +1464,9 @@ __global__ void __launch_bounds__(1024) k_plan(PlanArgs<typename KeyTraits<T>::K
   uint32_t* aidx = P + B;                              // active flag -> index + 1
   uint32_t* rk_bits = aidx + B;                        // rank words under construction (first pass)
   uint32_t* rk_first = rk_bits + rank_word_count(B);
+  uint32_t* xtra = rk_first + rank_word_count(B);  // the rest of the launch's LDS
+  const uint32_t used = 2 * B + 2 * rank_word_count(B);
+  const uint32_t xtra_words = a.lds_words > used ? a.lds_words - used : 0u;
   uint32_t* kfill = P;  // from P5 on (P and its reuse as range bases are dead): ks per small segment
 
   __shared__ uint32_t ord_cnt[kOrderBins];  // small segments per size class (finish order)
@@ -1597,11 +1601,12 @@ __global__ void __launch_bounds__(1024) k_plan(PlanArgs<typename KeyTraits<T>::K
   if (Bn > a.B_cap) Bn = a.B_cap;
 
   // P5: slot tables, small segment descriptors, next ranges (unplaced).
-  // With room in LDS (behind kfill in P, and aidx once P7 is done) the small
-  // segments' class information and k-list starts are kept there for P8, so
-  // it does not read back what this workgroup just wrote to global memory.
-  const bool lcache = 2 * U_small <= B && U_small < B;
-  uint32_t* sinfo = P + U_small;  // class information per small segment
+  // With room in LDS (past the tables, or behind kfill in P; and aidx once P7
+  // is done) the small segments' class information and k-list starts are kept
+  // there for P8, so it does not read back what this workgroup just wrote to
+  // global memory.
+  const bool lcache = U_small < B && (U_small <= xtra_words || 2 * U_small <= B);
+  uint32_t* sinfo = U_small <= xtra_words ? xtra : P + U_small;  // class information per small segment
   uint32_t* kst = aidx;           // k-list start per small segment (from P8 on)
   uint32_t small_max = 0;  // this thread's longest small segment
   // four active buckets per thread at a time, their loads issued together
