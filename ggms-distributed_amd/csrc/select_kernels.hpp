@@ -3014,15 +3014,47 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8)
   // into it clears it again once done
   select_zero_first(L);
   __syncthreads();
-  for (uint32_t jj = jj_lo + blockIdx.x; jj < jj_hi; jj += gridDim.x) {
-    const uint32_t j = small_order ? small_order[jj] : jj;
-    const uint32_t st = small_start[j], len = small_len[j];
+  // The next segment's descriptor (and the order entry of the one after it)
+  // is read while this one is finished, so a segment starts with its own
+  // loads instead of two dependent round trips for where it lies (4-byte
+  // keys: with 8-byte keys the held descriptor spills and costs more than
+  // it saves).
+  constexpr bool kPre = sizeof(K) == 4;
+  struct Seg {
+    uint32_t st, len, q0, q1;
+    K klo, khi;
+  };
+  auto order_of = [&](uint32_t x) { return small_order ? small_order[x] : x; };
+  auto seg_of = [&](uint32_t j) {
+    Seg g;
+    g.st = small_start[j];
+    g.len = small_len[j];
+    g.klo = small_klo[j];
+    g.khi = small_khi[j];
+    g.q0 = small_kstart[j];
+    g.q1 = small_kstart[j + 1];
+    return g;
+  };
+  const uint32_t gs = gridDim.x;
+  Seg sn{};
+  uint32_t jn2 = 0;
+  if (kPre) {
+    if (jj_lo + blockIdx.x < jj_hi) sn = seg_of(order_of(jj_lo + blockIdx.x));
+    if (jj_lo + blockIdx.x + gs < jj_hi) jn2 = order_of(jj_lo + blockIdx.x + gs);
+  }
+  for (uint32_t jj = jj_lo + blockIdx.x; jj < jj_hi; jj += gs) {
+    const Seg g = kPre ? sn : seg_of(order_of(jj));
+    if (kPre) {
+      if (jj + gs < jj_hi) sn = seg_of(jn2);
+      if (jj + 2 * gs < jj_hi) jn2 = order_of(jj + 2 * gs);
+    }
+    const uint32_t st = g.st, len = g.len;
     if (len > fin_cap) {  // cannot happen: such segments are ordered into k_finish_wide's range; reported
       if (threadIdx.x == 0) *err = 1u;  // a flag word in host-visible memory
       continue;
     }
-    const K klo = small_klo[j], khi = small_khi[j];
-    const uint32_t q0 = small_kstart[j], q1 = small_kstart[j + 1];
+    const K klo = g.klo, khi = g.khi;
+    const uint32_t q0 = g.q0, q1 = g.q1;
     if (!finish_fits_half<K>(len, klo, khi, fin_cap)) continue;  // k_finish_wide's
     // the ks and their ranks are fetched while the segment loads
     uint32_t kk = 0, kr = 0;
