@@ -497,8 +497,9 @@ __host__ __device__ __forceinline__ bool finish_fits_half(uint32_t len, K klo, K
 // and kFinishWaveKs ks, finished one per wave, then class 0, ordered last, for
 // the segments selected from global memory (too wide and too large for the
 // LDS layouts, or larger than the finish capacity: the one-pass
-// bucketMultiselect finishes every k-bucket whatever its size).  The one-wave class is used only
-// when a pass leaves at least kFinishWaveMinSegs small segments: a wave
+// bucketMultiselect finishes every k-bucket whatever its size).  With 4-byte
+// keys a segment of up to kFinishWaveLen4 keys and one k is one-wave too.  The
+// one-wave class is used only when a pass leaves at least kFinishWaveMinSegs small segments: a wave
 // sweeping its segment from L2 is slower per segment than a workgroup with
 // the segment in LDS (2.6 K keys: ~35 vs ~12 us), and wins only once the
 // segments outnumber the workgroups the GPU holds at once (8163 segments:
@@ -507,6 +508,11 @@ constexpr uint32_t kOrderBins = 35;
 constexpr uint32_t kFinishWaveLen = 4096;
 constexpr uint32_t kFinishWaveKs = 4;
 constexpr uint32_t kFinishWaveMinSegs = 2048;
+// 4-byte keys: segments of up to kFinishWaveLen4 keys with one k too (|N(0,1)|
+// 2^26, 8192 even ks, ~5.5 K keys a segment: finish 0.170 -> 0.089 ms; mixture
+// 2^28 8192 ks: 0.318 -> 0.299 ms; with up to 2 ks, or up to 12288 keys, the
+// mixture's finish is slower than without the longer form)
+constexpr uint32_t kFinishWaveLen4 = 8192;
 // With at least kFinishSplitMinSegs whole-workgroup segments below
 // kFinishSplitLen keys, the host-driven schedule finishes those with 512-thread
 // workgroups and half the LDS (three per CU instead of two) in a launch of
@@ -519,15 +525,19 @@ __host__ __device__ __forceinline__ bool finish_wide_segment(uint32_t len, K klo
 }
 // The part of a segment's class that its length and key interval decide:
 // low 8 bits the class without the one-wave option, bit 8 set when its
-// length allows the one-wave class; order_bin adds the ks' count.
+// length allows the one-wave class, bit 9 when the longer 4-byte-key form
+// does; order_bin adds the ks' count.
 template <class K>
 __host__ __device__ __forceinline__ uint32_t order_bin_info(uint32_t len, K klo, K khi, uint32_t fin_cap) {
   if (finish_wide_segment<K>(len, klo, khi, fin_cap)) return 0;
   const uint32_t base = len >> 10 < kOrderBins - 3 ? 2 + (len >> 10) : kOrderBins - 1;
-  return base | (len <= kFinishWaveLen ? 0x100u : 0u);
+  uint32_t wave = len <= kFinishWaveLen ? 0x100u : 0u;
+  if (sizeof(K) == 4 && len <= kFinishWaveLen4) wave |= 0x200u;
+  return base | wave;
 }
 __host__ __device__ __forceinline__ uint32_t order_bin_of(uint32_t info, uint32_t nks, bool waves) {
-  return (waves && (info & 0x100u) && nks <= kFinishWaveKs) ? 1u : (info & 0xFFu);
+  const bool wave = ((info & 0x100u) && nks <= kFinishWaveKs) || ((info & 0x200u) && nks == 1);
+  return (waves && wave) ? 1u : (info & 0xFFu);
 }
 template <class K>
 __host__ __device__ __forceinline__ uint32_t order_bin(uint32_t len, K klo, K khi, uint32_t fin_cap, uint32_t nks,
