@@ -105,7 +105,12 @@ static Config resolve_config(ggms_alg alg, uint64_t n, uint32_t nk, const ggms_o
   // the planning kernel and the column sums grow faster than pass 1 shrinks.
   // Measured at 2^28 fp32, 1001 ks, linear splitter: 8192 -> 0.802 ms,
   // 12288 -> 0.757, 14336 -> 0.750, 16384 -> 0.780.
-  c.B = o.num_buckets ? o.num_buckets : 14336u;
+  // With the segmented first pass on a large input, 16384: fewer survivors
+  // to move twice and to finish outweigh the larger plan and column sums
+  // (8192 random ks, 14336 -> 16384: 2^28 fp32 mixture 2.20 -> 2.11 ms linear,
+  // 1.739 -> 1.730 sketch; 2^27 fp64 N(0,1) 1.82 -> 1.73 ms; at 2^26,
+  // |N(0,1)| with 8192 even ks, 0.661 -> 0.672 ms, so not below 2^27 keys).
+  c.B = o.num_buckets ? o.num_buckets : (c.seg_first && !c.one_pass && n >= (1ull << 27) ? 16384u : 14336u);
   c.B = std::max<uint32_t>(16, std::min<uint32_t>(c.B, kMaxBuckets));
   c.B_cap = kMaxBuckets;
   if (c.mode == kModeLinear || c.mode == kModeStrided) {
