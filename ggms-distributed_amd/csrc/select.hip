@@ -110,7 +110,14 @@ static Config resolve_config(ggms_alg alg, uint64_t n, uint32_t nk, const ggms_o
   // (8192 random ks, 14336 -> 16384: 2^28 fp32 mixture 2.20 -> 2.11 ms linear,
   // 1.739 -> 1.730 sketch; 2^27 fp64 N(0,1) 1.82 -> 1.73 ms; at 2^26,
   // |N(0,1)| with 8192 even ks, 0.661 -> 0.672 ms, so not below 2^27 keys).
-  c.B = o.num_buckets ? o.num_buckets : (c.seg_first && !c.one_pass && n >= (1ull << 27) ? 16384u : 14336u);
+  // The same calls default to 128 quantile pieces and one first-pass tile per
+  // CU (fewer histogram rows to sum; the grouped first pass runs one workgroup
+  // per CU anyway): mixture 2.09 -> 2.06 ms linear, 1.719 -> 1.658 sketch,
+  // 1.730 -> 1.670 learned; 2^27 fp32 N(0,1) 0.980 -> 0.945 ms, fp64 1.732 ->
+  // 1.701 (one tile per CU makes the headline, 2^28 with 1001 ks, 0.68 -> 0.84
+  // ms and |N(0,1)| at 2^26 0.655 -> 0.679).
+  const bool big_seg = c.seg_first && !c.one_pass && n >= (1ull << 27);
+  c.B = o.num_buckets ? o.num_buckets : (big_seg ? 16384u : 14336u);
   c.B = std::max<uint32_t>(16, std::min<uint32_t>(c.B, kMaxBuckets));
   c.B_cap = kMaxBuckets;
   if (c.mode == kModeLinear || c.mode == kModeStrided) {
@@ -118,7 +125,7 @@ static Config resolve_config(ggms_alg alg, uint64_t n, uint32_t nk, const ggms_o
     c.P = std::max<uint32_t>(2, std::min<uint32_t>(c.P, 1024));
     c.S = o.sample_size ? o.sample_size : 1024;
   } else {
-    c.P = o.learned_pieces ? o.learned_pieces : 64;
+    c.P = o.learned_pieces ? o.learned_pieces : (big_seg ? 128u : 64u);
     c.P = std::max<uint32_t>(2, std::min<uint32_t>(c.P, kMaxPieces));
     c.S = o.sample_size ? o.sample_size : 8192;
     c.S = std::max<uint32_t>(c.S, 2 * kMlpWin);
@@ -131,7 +138,7 @@ static Config resolve_config(ggms_alg alg, uint64_t n, uint32_t nk, const ggms_o
   c.B0_bound = c.B + 2 * c.E + 4 + Q;
   c.NT = o.block_threads ? o.block_threads : 1024;
   if (c.NT % 64 || c.NT < 64 || c.NT > 1024) c.NT = 1024;
-  uint32_t bpc = o.blocks_per_cu ? o.blocks_per_cu : 2;
+  uint32_t bpc = o.blocks_per_cu ? o.blocks_per_cu : (big_seg ? 1u : 2u);
   c.G_max = std::max<uint32_t>(1, std::min<uint32_t>(bpc * (uint32_t)cus, kMaxWorkgroups));
   // passes >= 1: one workgroup per CU (their LDS tables and scattered writes
   // favour it: pass-1 count 0.059 -> 0.052 ms, compaction 0.085 -> 0.076 ms)
@@ -1307,7 +1314,7 @@ void ggms_opts_default(ggms_opts* o) {
   o->block_threads = 1024;
   o->blocks_per_cu = 0;
   o->splitter = GGMS_SPLIT_LINEAR;
-  o->learned_pieces = 64;
+  o->learned_pieces = 0;  // automatic (see resolve_config)
   o->max_passes = 64;
   o->seed = 0x5EEDull;
   o->stream = nullptr;
