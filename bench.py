@@ -273,8 +273,10 @@ def main():
         "dtype": "f32",
         "data": "synthetic (Philox4x32-10 seeded generator, Box-Muller normal)",
         "config": {"workload": f"iterativeSMOS, N=2^{args.log2n} fp32 {args.dist}(0,1), {args.nk} ks uniform "
-                               f"random in [1,N], linear splitter (the reference's 1024 samples and 17 pivots; "
-                               f"14336 first-pass buckets, the reference uses 8192: see linear_8192_buckets)",
+                               f"random in [1,N], linear splitter (the reference's 17 pivots over a "
+                               f"{2048 if n >= 1 << 27 and args.nk * 4 < 14336 else 1024}-key sample, the reference "
+                               f"draws 1024; 14336 first-pass buckets, the reference uses 8192: see "
+                               f"linear_8192_buckets)",
                    "n": n, "nk": args.nk, "seed": args.seed, "alg": "iterative", "splitter": "linear"},
         "roofline": roofline,
         "passes": passes,

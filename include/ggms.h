@@ -59,7 +59,7 @@ typedef enum {
 typedef struct {
   uint32_t num_buckets;   /* buckets per pass (0 = auto: 14336, or 16384 when the first pass is segmented and n >= 2^27; max 16384; the reference uses 8192) */
   uint32_t num_pivots;    /* linear splitter pivots (default 17 => 16 ranges) */
-  uint32_t sample_size;   /* splitter sample (default 1024; 8192 for the one-GPU distributed form) */
+  uint32_t sample_size;   /* splitter sample (default 1024, 2048 for the linear splitter on 4-byte keys with n >= 2^27 and few ks; 8192 for the one-GPU distributed form) */
   uint32_t block_threads; /* workgroup size of the streaming kernels (default 1024) */
   uint32_t blocks_per_cu; /* streaming workgroups per CU (0 = auto: 2 in the first pass, 1 after; 1 throughout when the first pass is segmented and n >= 2^27) */
   uint32_t splitter;      /* ggms_splitter (default LINEAR) */
