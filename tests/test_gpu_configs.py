@@ -58,7 +58,7 @@ def test_mixture_2p28_8192ks(gpu_ctx):
     n = 1 << 28
     x = ggms.generate("float32", "mixture", n, 505)
     ks = [ggms.generate_ks("uniform_random", 8192, n, 506)]
-    _check_all(gpu_ctx, x, ks, ("iterative",), ("linear", "learned"), "mixture 2^28")
+    _check_all(gpu_ctx, x, ks, ("iterative",), ("linear", "sketch", "learned"), "mixture 2^28")
 
 
 def test_fp64_normal_2p27_1001ks(gpu_ctx):
@@ -67,6 +67,16 @@ def test_fp64_normal_2p27_1001ks(gpu_ctx):
     x = ggms.generate("float64", "normal", n, 207)
     ks = [ggms.generate_ks("uniform_random", 1001, n, 208)]
     _check_all(gpu_ctx, x, ks, ("iterative", "distributed"), ("linear",), "fp64 normal 2^27")
+
+
+def test_fp64_normal_2p27_8192ks(gpu_ctx):
+    """8-byte keys with many ks on a large input: the segmented first pass with its
+    large-call defaults (16384 buckets, one tile per CU, 128 quantile pieces)."""
+    import ggms
+    n = 1 << 27
+    x = ggms.generate("float64", "normal", n, 209)
+    ks = [ggms.generate_ks("uniform_random", 8192, n, 210)]
+    _check_all(gpu_ctx, x, ks, ("iterative", "distributed"), ("linear", "sketch"), "fp64 normal 2^27 8192 ks")
 
 
 def test_headline_2p28_normal_1001ks(gpu_ctx):
